@@ -1,0 +1,191 @@
+/*
+ * mlpcg.h -- C ABI of the MI355X (gfx950) neural-preconditioned PCG library
+ * (libmlpcg_hip.so).
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream (hipStream_t
+ * passed as void*, NULL = default stream), launches asynchronously on that stream,
+ * and returns 0 on success or -1 with a message available from mlpcg_last_error()
+ * (thread-local).  No entry point synchronises the device unless stated.
+ *
+ * Conventions
+ *   grid (n1, n2, n3) with the last axis fastest; V = n1*n2*n3.
+ *   image  : [C][V] contiguous, C in {1, 2, 3}
+ *   x, y   : [B][V] contiguous (the reference's (B, 1, n1, n2, n3))
+ *   weight : [27][C][3][3][3], bias : [27] for SM blocks; [1][C][3][3][3] and [1]
+ *            for SM linear -- in the ACCUMULATE type: double for MLPCG_F64, float
+ *            for MLPCG_F32 / MLPCG_F16 / MLPCG_BF16.
+ *   Values outside the grid are zero.  Unlike the reference, inputs are NOT
+ *   pre-padded: the one-voxel halo is handled inside the kernels.
+ *   Device pointers must be 16-byte aligned.
+ */
+#ifndef MLPCG_H
+#define MLPCG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLPCG_F64 0
+#define MLPCG_F32 1
+#define MLPCG_F16 2
+#define MLPCG_BF16 3
+
+#define MLPCG_NPCG 0      /* cg_tests.py:26   npcg      */
+#define MLPCG_NPCG_FLEX 1 /* cg_tests.py:103  npcg_flex */
+#define MLPCG_NPSD 2      /* cg_tests.py:189  npsd      */
+
+#define MLPCG_MODEL_SMALL_SM 0  /* SmallSMModelDn3D    sm_model_3d.py:164 */
+#define MLPCG_MODEL_SPD 1       /* SPDSMModelDn3D      sm_model_3d.py:293 */
+#define MLPCG_MODEL_SMALL_SPD 2 /* SmallSPDSMModelDn3D sm_model_3d.py:388 */
+
+typedef struct mlpcg_precond mlpcg_precond;
+typedef struct mlpcg_solver mlpcg_solver;
+
+typedef struct {
+  int iters;       /* preconditioner applications, counted as cg_tests.py returns k */
+  int status;      /* 0 converged, 1 max_it reached, 2 non-finite residual */
+  double rel_res;  /* ||b - A x|| / ||b|| of the last FP64 residual recompute */
+  double norm_b;   /* ||b|| */
+} mlpcg_solve_info;
+
+/* ---- library ------------------------------------------------------------------ */
+const char* mlpcg_last_error(void);
+int mlpcg_version(void);
+
+/* ---- SM block (replaces torch_extension/sm_block_3d.cpp) -------------------------
+ * y[b,v] = sum_p K_p(v) x[b, v+d_p],  K_p(v) = bias[p] + sum_{s,e} weight[p,s,e] image[s, v+e].
+ * workspace: mlpcg_sm_block_workspace_size() bytes (stencil weights + gradient partials). */
+size_t mlpcg_sm_block_workspace_size(int C, int B, int n1, int n2, int n3, int dtype);
+/* sm_block_3d.cpp:31  smblock3d.inference(image, x, weights, bias) -> [y] */
+int mlpcg_sm_block_inference(const void* image, const void* x, const void* weight, const void* bias,
+                             void* y, void* workspace, int C, int B, int n1, int n2, int n3,
+                             int dtype, void* stream);
+/* sm_block_3d.cpp:44  smblock3d.forward(image, x, weights, bias) -> [y] */
+int mlpcg_sm_block_forward(const void* image, const void* x, const void* weight, const void* bias,
+                           void* y, void* workspace, int C, int B, int n1, int n2, int n3, int dtype,
+                           void* stream);
+/* sm_block_3d.cpp:57  smblock3d.backward(grad_output, image, x, weights, bias) -> [gx, gW, gb]
+ * grad_x = transpose stencil of grad_output; grad_w/grad_b in the accumulate type. */
+int mlpcg_sm_block_backward(const void* grad_out, const void* image, const void* x,
+                            const void* weight, const void* bias, void* grad_x, void* grad_w,
+                            void* grad_b, void* workspace, int C, int B, int n1, int n2, int n3,
+                            int dtype, void* stream);
+
+/* ---- SM block transpose (replaces torch_extension/sm_block_trans_3d.cpp) ----------
+ * y[b,v] = sum_p K_p(v - d_p) x[b, v - d_p]. */
+/* sm_block_trans_3d.cpp:30  smblocktrans3d.inference */
+int mlpcg_sm_block_trans_inference(const void* image, const void* x, const void* weight,
+                                   const void* bias, void* y, void* workspace, int C, int B, int n1,
+                                   int n2, int n3, int dtype, void* stream);
+/* sm_block_trans_3d.cpp:43  smblocktrans3d.forward */
+int mlpcg_sm_block_trans_forward(const void* image, const void* x, const void* weight,
+                                 const void* bias, void* y, void* workspace, int C, int B, int n1,
+                                 int n2, int n3, int dtype, void* stream);
+/* sm_block_trans_3d.cpp:56  smblocktrans3d.backward -> [gx, gW, gb] */
+int mlpcg_sm_block_trans_backward(const void* grad_out, const void* image, const void* x,
+                                  const void* weight, const void* bias, void* grad_x, void* grad_w,
+                                  void* grad_b, void* workspace, int C, int B, int n1, int n2,
+                                  int n3, int dtype, void* stream);
+
+/* ---- SM linear (replaces torch_extension/sm_linear_3d.cpp) ------------------------
+ * ybar[s,e] = (1/V) sum_v image[s, v+e];  z = sum w*ybar + b.  z ([1]) and ybar
+ * ([C*27]) are in the accumulate type. */
+size_t mlpcg_sm_linear_workspace_size(int C, int n1, int n2, int n3, int dtype);
+/* sm_linear_3d.cpp:36  smlinear3d.inference(image, weights, bias) -> [z, ybar] */
+int mlpcg_sm_linear_inference(const void* image, const void* weight, const void* bias, void* z,
+                              void* ybar, void* workspace, int C, int n1, int n2, int n3,
+                              int dtype, void* stream);
+/* sm_linear_3d.cpp:25  smlinear3d.forward(image, weights, bias) -> [z, ybar] */
+int mlpcg_sm_linear_forward(const void* image, const void* weight, const void* bias, void* z,
+                            void* ybar, void* workspace, int C, int n1, int n2, int n3, int dtype,
+                            void* stream);
+/* sm_linear_3d.cpp:47  smlinear3d.backward(grad_output, ybar) -> [gw, gb] */
+int mlpcg_sm_linear_backward(const void* grad_z, const void* ybar, void* grad_w, void* grad_b,
+                             int C, int dtype, void* stream);
+
+/* ---- building blocks of the streamed-stencil path --------------------------------
+ * No reference counterpart as separate calls: the reference recomputes K inside each
+ * block kernel (sm_block_3d_kernel.cu:46-69) and calls F.avg_pool3d / F.interpolate
+ * (sm_model_3d.py:206-230). */
+/* K[27][V] (transposed != 0: Kt_q(v) = K_{26-q}(v + d_q), so the transpose block is
+ * mlpcg_sm_apply with Kt).  image_dtype in {K_dtype, MLPCG_F32}. */
+int mlpcg_stencil_weights(const void* image, int image_dtype, const void* weight, const void* bias,
+                          void* K, int K_dtype, int transposed, int C, int n1, int n2, int n3,
+                          void* stream);
+/* y = epilogue(sum_p K_p x(.+d_p)): *in_scale multiplies the sum (FP64 device scalar),
+ * mask_a zeroes where 0, then y = (*c0) prev + (*c1) y if prev != NULL, then mask_b
+ * zeroes where 0.  Type combinations: all K_dtype, or x F32 / y K_dtype, or x K_dtype /
+ * y F32 (K_dtype in {F16, BF16}). c0/c1 are accumulate-type device scalars. */
+int mlpcg_sm_apply(const void* K, int K_dtype, const void* x, int x_dtype, void* y, int y_dtype,
+                   const void* prev, const void* c0, const void* c1, const uint8_t* mask_a,
+                   const uint8_t* mask_b, const double* in_scale, int B, int n1, int n2, int n3,
+                   void* stream);
+/* F.avg_pool3d(x, 2): (n1, n2, n3) are the FINE dims (all even). */
+int mlpcg_avgpool2(const void* in, void* out, int B, int n1, int n2, int n3, int dtype,
+                   void* stream);
+/* out = scale * F.interpolate(in, 2, trilinear|nearest, align_corners=False), or
+ * out = (*c0) prev + (*c1) scale * up(in) when prev != NULL; FINE dims. */
+int mlpcg_upsample2(const void* in, void* out, int B, int n1, int n2, int n3, int nearest,
+                    double scale, const void* prev, const void* c0, const void* c1, int dtype,
+                    void* stream);
+/* Adjoint (transpose) of mlpcg_upsample2 without prev; COARSE dims. */
+int mlpcg_upsample2_adjoint(const void* grad_fine, void* grad_coarse, int B, int n1, int n2,
+                            int n3, int nearest, double scale, int dtype, void* stream);
+
+/* ---- fluid Laplacian and dots (replaces torch sparse CSR A @ p, cg_tests.py:53) ---- */
+/* code[v] = flags[v] == FLUID ? 1 + #non-solid face neighbours : 0 (flag codes of
+ * lib/read_data.py:13-16; outside the grid counts as solid). */
+int mlpcg_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3, void* stream);
+/* y = A x over the full grid, matrix-free from code; x must vanish off the unknowns. */
+int mlpcg_spmv_flags(const uint8_t* code, const float* x, float* y, int n1, int n2, int n3,
+                     void* stream);
+/* y = A x, CSR with int32 indices; dtype MLPCG_F32 or MLPCG_F64 (val, x, y alike). */
+int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const void* x, void* y,
+                   long n, int dtype, void* stream);
+/* *out = sum x[i] y[i] accumulated in FP64 (device scalar). */
+size_t mlpcg_dot_workspace_size(void);
+int mlpcg_dot(const float* x, const float* y, long n, double* out, void* workspace, void* stream);
+
+/* ---- preconditioner plan (SmallSMModelDn3D.eval_forward with K cached) ------------
+ * weights/biases: host arrays of device pointers in state_dict order:
+ *   SMALL_SM : pre.0..n-1, post.0..n-1, l, c0.0..n-1, c1.0..n-1      (4n+1)
+ *   SPD      : l0.0..n-1,  l1.0..n-1,  l, c0.0..n-1, c1.0..n-1       (4n+1)
+ *   SMALL_SPD: l0.0..n-1,  l, c0.0..n-1, c1.0..n-1                   (3n+1)
+ * image: [C][V] in the accumulate type.  dtype is the storage type of the stencils
+ * and level vectors.  Allocates device memory (released by destroy).
+ * apply: z = P(r * (*in_scale)) with r, z F32 (F64 for an F64 plan) on the full grid,
+ * zeroed where out_mask == 0 if out_mask != NULL. */
+int mlpcg_precond_create(mlpcg_precond** out, int kind, int n_levels, int C, int n1, int n2,
+                         int n3, int dtype, int nearest, int mask, int swap, const void* image,
+                         const void* const* weights, const void* const* biases, void* stream);
+int mlpcg_precond_apply(mlpcg_precond* P, const void* r, void* z, const double* in_scale,
+                        const uint8_t* out_mask, void* stream);
+size_t mlpcg_precond_bytes(const mlpcg_precond* P);
+void mlpcg_precond_destroy(mlpcg_precond* P);
+
+/* ---- PCG solver (cg_tests.py npcg / npcg_flex / npsd, test.py:42-51 predict) ------
+ * flags solver: vectors live on the full grid (length V, zero off the fluid cells),
+ * A is applied matrix-free.  CSR solver: vectors over the n unknowns, P is applied on
+ * the grid through scatter/gather with fluid_idx (borrowed; must outlive the solver).
+ * solve: stops when ||r||/||b|| <= max(tol, atol/||b||) (npsd: <) or after max_it
+ * preconditioner applications; r is recomputed as b - A x in FP64 every iteration;
+ * P == NULL means no preconditioning.  history (host, max_it+1 doubles) receives the
+ * relative residuals if not NULL.  Synchronises the stream before returning. */
+int mlpcg_solver_create_flags(mlpcg_solver** out, const int8_t* flags, int n1, int n2, int n3,
+                              void* stream);
+int mlpcg_solver_create_csr(mlpcg_solver** out, const int* rowptr, const int* col,
+                            const float* val, long n, const int* fluid_idx, int n1, int n2, int n3,
+                            void* stream);
+int mlpcg_solver_solve(mlpcg_solver* S, mlpcg_precond* P, const float* b, const float* x0,
+                       float* x, int method, int max_it, double tol, double atol,
+                       double* history, mlpcg_solve_info* info, void* stream);
+void mlpcg_solver_destroy(mlpcg_solver* S);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MLPCG_H */

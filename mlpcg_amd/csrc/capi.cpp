@@ -1,0 +1,350 @@
+// extern "C" entry points of libmlpcg_hip.so (declared in include/mlpcg.h).
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/mlpcg.h"
+#include "runtime.hpp"
+
+namespace mlpcg {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: kernel launch failed: %s", what, hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+static size_t elem_size(int dt) { return dt == kF64 ? 8 : (dt == kF32 ? 4 : 2); }
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace mlpcg
+
+using namespace mlpcg;
+
+struct mlpcg_precond {
+  Precond* impl;
+};
+struct mlpcg_solver {
+  Solver* impl;
+};
+
+#define STREAM(s) ((hipStream_t)(s))
+
+static int check_block_args(const char* who, int C, int B, int n1, int n2, int n3, int dtype) {
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "%s: C must be 1, 2 or 3 (got %d)", who, C);
+  MLPCG_REQUIRE(B >= 1 && n1 >= 1 && n2 >= 1 && n3 >= 1, "%s: bad sizes B=%d grid=(%d,%d,%d)", who, B,
+                n1, n2, n3);
+  MLPCG_REQUIRE(dtype >= kF64 && dtype <= kBF16, "%s: bad dtype %d", who, dtype);
+  MLPCG_REQUIRE((long)n1 * n2 * n3 * 27 < (1L << 31) * 4, "%s: grid too large", who);
+  return 0;
+}
+
+extern "C" {
+
+const char* mlpcg_last_error(void) { return g_err; }
+int mlpcg_version(void) { return 100; }
+
+// --------------------------------------------------------------------------- SM block
+size_t mlpcg_sm_block_workspace_size(int C, int B, int n1, int n2, int n3, int dtype) {
+  const size_t V = (size_t)n1 * n2 * n3;
+  size_t k = (27 * V * elem_size(dtype) + 255) & ~(size_t)255;
+  return k + weight_grad_workspace(dtype, C, B, n1, n2, n3);
+}
+
+static int block_fwd(bool trans, const void* image, const void* x, const void* weight,
+                     const void* bias, void* y, void* workspace, int C, int B, int n1, int n2,
+                     int n3, int dtype, void* stream) {
+  const char* who = trans ? "sm_block_trans" : "sm_block";
+  if (check_block_args(who, C, B, n1, n2, n3, dtype)) return -1;
+  MLPCG_REQUIRE(image && x && weight && bias && y && workspace, "%s: null pointer", who);
+  MLPCG_REQUIRE(aligned16(x) && aligned16(y) && aligned16(workspace) && aligned16(image),
+                "%s: pointers must be 16-byte aligned", who);
+  if (launch_stencil_weights(dtype, image, weight, bias, workspace, C, n1, n2, n3, trans, STREAM(stream)))
+    return -1;
+  ApplyArgs a;
+  memset(&a, 0, sizeof(a));
+  a.K = workspace;
+  a.x = x;
+  a.y = y;
+  a.B = B;
+  a.n1 = n1;
+  a.n2 = n2;
+  a.n3 = n3;
+  return launch_sm_apply(dtype, dtype, dtype, a, STREAM(stream));
+}
+
+static int block_bwd(bool trans, const void* g, const void* image, const void* x, const void* weight,
+                     const void* bias, void* gx, void* gw, void* gb, void* workspace, int C, int B,
+                     int n1, int n2, int n3, int dtype, void* stream) {
+  const char* who = trans ? "sm_block_trans_backward" : "sm_block_backward";
+  if (check_block_args(who, C, B, n1, n2, n3, dtype)) return -1;
+  MLPCG_REQUIRE(g && image && x && weight && bias && gx && gw && gb && workspace, "%s: null pointer", who);
+  MLPCG_REQUIRE(aligned16(g) && aligned16(x) && aligned16(gx) && aligned16(workspace) && aligned16(image),
+                "%s: pointers must be 16-byte aligned", who);
+  const size_t V = (size_t)n1 * n2 * n3;
+  // the input gradient is the opposite stencil: transpose for the block, block for the
+  // transpose (stencil weights recomputed here, not saved from the forward pass)
+  if (launch_stencil_weights(dtype, image, weight, bias, workspace, C, n1, n2, n3, !trans, STREAM(stream)))
+    return -1;
+  ApplyArgs a;
+  memset(&a, 0, sizeof(a));
+  a.K = workspace;
+  a.x = g;
+  a.y = gx;
+  a.B = B;
+  a.n1 = n1;
+  a.n2 = n2;
+  a.n3 = n3;
+  if (launch_sm_apply(dtype, dtype, dtype, a, STREAM(stream))) return -1;
+  void* part = (char*)workspace + ((27 * V * elem_size(dtype) + 255) & ~(size_t)255);
+  // block: G_p(v) = sum_b g[b,v] x[b,v+d_p]; transpose: H_p(u) = sum_b x[b,u] g[b,u+d_p]
+  const void* av = trans ? x : g;
+  const void* cv = trans ? g : x;
+  return launch_weight_grad(dtype, av, cv, image, gw, gb, part, C, B, n1, n2, n3, STREAM(stream));
+}
+
+int mlpcg_sm_block_inference(const void* image, const void* x, const void* weight, const void* bias,
+                             void* y, void* workspace, int C, int B, int n1, int n2, int n3,
+                             int dtype, void* stream) {
+  return block_fwd(false, image, x, weight, bias, y, workspace, C, B, n1, n2, n3, dtype, stream);
+}
+int mlpcg_sm_block_forward(const void* image, const void* x, const void* weight, const void* bias,
+                           void* y, void* workspace, int C, int B, int n1, int n2, int n3, int dtype,
+                           void* stream) {
+  return block_fwd(false, image, x, weight, bias, y, workspace, C, B, n1, n2, n3, dtype, stream);
+}
+int mlpcg_sm_block_backward(const void* grad_out, const void* image, const void* x,
+                            const void* weight, const void* bias, void* grad_x, void* grad_w,
+                            void* grad_b, void* workspace, int C, int B, int n1, int n2, int n3,
+                            int dtype, void* stream) {
+  return block_bwd(false, grad_out, image, x, weight, bias, grad_x, grad_w, grad_b, workspace, C, B, n1,
+                   n2, n3, dtype, stream);
+}
+int mlpcg_sm_block_trans_inference(const void* image, const void* x, const void* weight,
+                                   const void* bias, void* y, void* workspace, int C, int B, int n1,
+                                   int n2, int n3, int dtype, void* stream) {
+  return block_fwd(true, image, x, weight, bias, y, workspace, C, B, n1, n2, n3, dtype, stream);
+}
+int mlpcg_sm_block_trans_forward(const void* image, const void* x, const void* weight,
+                                 const void* bias, void* y, void* workspace, int C, int B, int n1,
+                                 int n2, int n3, int dtype, void* stream) {
+  return block_fwd(true, image, x, weight, bias, y, workspace, C, B, n1, n2, n3, dtype, stream);
+}
+int mlpcg_sm_block_trans_backward(const void* grad_out, const void* image, const void* x,
+                                  const void* weight, const void* bias, void* grad_x, void* grad_w,
+                                  void* grad_b, void* workspace, int C, int B, int n1, int n2,
+                                  int n3, int dtype, void* stream) {
+  return block_bwd(true, grad_out, image, x, weight, bias, grad_x, grad_w, grad_b, workspace, C, B, n1,
+                   n2, n3, dtype, stream);
+}
+
+// --------------------------------------------------------------------------- SM linear
+size_t mlpcg_sm_linear_workspace_size(int C, int n1, int n2, int n3, int dtype) {
+  return mean_patch_workspace(dtype, C, n1, n2, n3);
+}
+int mlpcg_sm_linear_inference(const void* image, const void* weight, const void* bias, void* z,
+                              void* ybar, void* workspace, int C, int n1, int n2, int n3,
+                              int dtype, void* stream) {
+  if (check_block_args("sm_linear", C, 1, n1, n2, n3, dtype)) return -1;
+  MLPCG_REQUIRE(image && weight && bias && z && ybar && workspace, "sm_linear: null pointer");
+  MLPCG_REQUIRE(aligned16(image), "sm_linear: image must be 16-byte aligned");
+  return launch_sm_linear(dtype, image, weight, bias, z, ybar, workspace, C, n1, n2, n3, STREAM(stream));
+}
+int mlpcg_sm_linear_forward(const void* image, const void* weight, const void* bias, void* z,
+                            void* ybar, void* workspace, int C, int n1, int n2, int n3, int dtype,
+                            void* stream) {
+  return mlpcg_sm_linear_inference(image, weight, bias, z, ybar, workspace, C, n1, n2, n3, dtype, stream);
+}
+int mlpcg_sm_linear_backward(const void* grad_z, const void* ybar, void* grad_w, void* grad_b, int C,
+                             int dtype, void* stream) {
+  MLPCG_REQUIRE(grad_z && ybar && grad_w && grad_b, "sm_linear_backward: null pointer");
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "sm_linear_backward: C must be 1, 2 or 3");
+  return launch_linear_backward(dtype, grad_z, ybar, grad_w, grad_b, C, STREAM(stream));
+}
+
+// --------------------------------------------------------------------------- building blocks
+int mlpcg_stencil_weights(const void* image, int image_dtype, const void* weight, const void* bias,
+                          void* K, int K_dtype, int transposed, int C, int n1, int n2, int n3,
+                          void* stream) {
+  if (check_block_args("stencil_weights", C, 1, n1, n2, n3, K_dtype)) return -1;
+  MLPCG_REQUIRE(image && weight && bias && K, "stencil_weights: null pointer");
+  MLPCG_REQUIRE(aligned16(image) && aligned16(K), "stencil_weights: pointers must be 16-byte aligned");
+  return launch_stencil_weights2(image_dtype, K_dtype, image, weight, bias, K, C, n1, n2, n3,
+                                 transposed != 0, STREAM(stream));
+}
+
+int mlpcg_sm_apply(const void* K, int K_dtype, const void* x, int x_dtype, void* y, int y_dtype,
+                   const void* prev, const void* c0, const void* c1, const uint8_t* mask_a,
+                   const uint8_t* mask_b, const double* in_scale, int B, int n1, int n2, int n3,
+                   void* stream) {
+  MLPCG_REQUIRE(K && x && y, "sm_apply: null pointer");
+  MLPCG_REQUIRE(!prev || (c0 && c1), "sm_apply: prev needs c0 and c1");
+  MLPCG_REQUIRE(aligned16(K) && aligned16(x) && aligned16(y) && (!prev || aligned16(prev)),
+                "sm_apply: pointers must be 16-byte aligned");
+  ApplyArgs a;
+  a.K = K;
+  a.x = x;
+  a.y = y;
+  a.prev = prev;
+  a.c0 = c0;
+  a.c1 = c1;
+  a.mask_a = mask_a;
+  a.mask_b = mask_b;
+  a.in_scale = in_scale;
+  a.done = nullptr;
+  a.B = B;
+  a.n1 = n1;
+  a.n2 = n2;
+  a.n3 = n3;
+  return launch_sm_apply(K_dtype, x_dtype, y_dtype, a, STREAM(stream));
+}
+
+int mlpcg_avgpool2(const void* in, void* out, int B, int n1, int n2, int n3, int dtype, void* stream) {
+  MLPCG_REQUIRE(in && out, "avgpool2: null pointer");
+  MLPCG_REQUIRE(B >= 1, "avgpool2: B must be >= 1");
+  return launch_avgpool2(dtype, in, out, B, n1, n2, n3, nullptr, STREAM(stream));
+}
+
+int mlpcg_upsample2(const void* in, void* out, int B, int n1, int n2, int n3, int nearest, double scale,
+                    const void* prev, const void* c0, const void* c1, int dtype, void* stream) {
+  MLPCG_REQUIRE(in && out, "upsample2: null pointer");
+  MLPCG_REQUIRE(!prev || (c0 && c1), "upsample2: prev needs c0 and c1");
+  UpArgs u;
+  u.in = in;
+  u.out = out;
+  u.prev = prev;
+  u.c0 = c0;
+  u.c1 = c1;
+  u.done = nullptr;
+  u.scale = scale;
+  u.nearest = nearest;
+  u.B = B;
+  u.n1 = n1;
+  u.n2 = n2;
+  u.n3 = n3;
+  return launch_upsample2(dtype, u, STREAM(stream));
+}
+
+int mlpcg_upsample2_adjoint(const void* grad_fine, void* grad_coarse, int B, int n1, int n2, int n3,
+                            int nearest, double scale, int dtype, void* stream) {
+  MLPCG_REQUIRE(grad_fine && grad_coarse, "upsample2_adjoint: null pointer");
+  return launch_upsample2_adjoint(dtype, grad_fine, grad_coarse, B, n1, n2, n3, nearest, scale,
+                                  STREAM(stream));
+}
+
+// --------------------------------------------------------------------------- Laplacian
+int mlpcg_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3, void* stream) {
+  MLPCG_REQUIRE(flags && code, "flags_code: null pointer");
+  return launch_flags_code(flags, code, n1, n2, n3, STREAM(stream));
+}
+
+int mlpcg_spmv_flags(const uint8_t* code, const float* x, float* y, int n1, int n2, int n3, void* stream) {
+  MLPCG_REQUIRE(code && x && y, "spmv_flags: null pointer");
+  MLPCG_REQUIRE(aligned16(code) && aligned16(x) && aligned16(y), "spmv_flags: pointers must be 16-byte aligned");
+  return launch_spmv_flat(code, x, y, n1, n2, n3, STREAM(stream));
+}
+
+int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const void* x, void* y, long n,
+                   int dtype, void* stream) {
+  MLPCG_REQUIRE(rowptr && col && val && x && y, "spmv_csr: null pointer");
+  if (dtype == kF32)
+    return launch_spmv_csr(rowptr, col, (const float*)val, (const float*)x, (float*)y, n, STREAM(stream));
+  MLPCG_REQUIRE(dtype == kF64, "spmv_csr: dtype must be F32 or F64");
+  return launch_spmv_csr_f64(rowptr, col, (const double*)val, (const double*)x, (double*)y, n,
+                             STREAM(stream));
+}
+
+size_t mlpcg_dot_workspace_size(void) { return reduction_workspace_bytes() + 256; }
+
+int mlpcg_dot(const float* x, const float* y, long n, double* out, void* workspace, void* stream) {
+  MLPCG_REQUIRE(x && y && out && workspace, "dot: null pointer");
+  // workspace: [counter (256 B, zero on first use; re-armed by the kernel)] [partials]
+  unsigned* counter = (unsigned*)workspace;
+  double* partials = (double*)((char*)workspace + 256);
+  return launch_dot(x, y, n, partials, counter, out, STREAM(stream));
+}
+
+// --------------------------------------------------------------------------- plan
+int mlpcg_precond_create(mlpcg_precond** out, int kind, int n_levels, int C, int n1, int n2, int n3,
+                         int dtype, int nearest, int mask, int swap, const void* image,
+                         const void* const* weights, const void* const* biases, void* stream) {
+  MLPCG_REQUIRE(out && image && weights && biases, "precond_create: null pointer");
+  Precond* P = nullptr;
+  if (precond_create(&P, kind, n_levels, C, n1, n2, n3, dtype, nearest, mask, swap, image, weights,
+                     biases, STREAM(stream)))
+    return -1;
+  *out = new mlpcg_precond{P};
+  return 0;
+}
+
+int mlpcg_precond_apply(mlpcg_precond* P, const void* r, void* z, const double* in_scale,
+                        const uint8_t* out_mask, void* stream) {
+  MLPCG_REQUIRE(P && P->impl && r && z, "precond_apply: null pointer");
+  return P->impl->apply((const float*)r, (float*)z, in_scale, out_mask, nullptr, STREAM(stream));
+}
+
+size_t mlpcg_precond_bytes(const mlpcg_precond* P) { return P && P->impl ? P->impl->bytes : 0; }
+
+void mlpcg_precond_destroy(mlpcg_precond* P) {
+  if (!P) return;
+  delete P->impl;
+  delete P;
+}
+
+// --------------------------------------------------------------------------- solver
+int mlpcg_solver_create_flags(mlpcg_solver** out, const int8_t* flags, int n1, int n2, int n3,
+                              void* stream) {
+  MLPCG_REQUIRE(out, "solver_create_flags: null output");
+  Solver* S = nullptr;
+  if (solver_create_flat(&S, flags, n1, n2, n3, STREAM(stream))) return -1;
+  *out = new mlpcg_solver{S};
+  return 0;
+}
+
+int mlpcg_solver_create_csr(mlpcg_solver** out, const int* rowptr, const int* col, const float* val,
+                            long n, const int* fluid_idx, int n1, int n2, int n3, void* stream) {
+  MLPCG_REQUIRE(out, "solver_create_csr: null output");
+  Solver* S = nullptr;
+  if (solver_create_csr(&S, rowptr, col, val, n, fluid_idx, n1, n2, n3, STREAM(stream))) return -1;
+  *out = new mlpcg_solver{S};
+  return 0;
+}
+
+int mlpcg_solver_solve(mlpcg_solver* S, mlpcg_precond* P, const float* b, const float* x0, float* x,
+                       int method, int max_it, double tol, double atol, double* history,
+                       mlpcg_solve_info* info, void* stream) {
+  MLPCG_REQUIRE(S && S->impl, "solver_solve: null solver");
+  SolveInfo si;
+  memset(&si, 0, sizeof(si));
+  const int rc = S->impl->solve(P ? P->impl : nullptr, b, x0, x, method, max_it, tol, atol, history,
+                                &si, STREAM(stream));
+  if (rc) return rc;
+  if (info) {
+    info->iters = si.iters;
+    info->status = si.status;
+    info->rel_res = si.rel_res;
+    info->norm_b = si.norm_b;
+  }
+  return 0;
+}
+
+void mlpcg_solver_destroy(mlpcg_solver* S) {
+  if (!S) return;
+  delete S->impl;
+  delete S;
+}
+
+}  // extern "C"

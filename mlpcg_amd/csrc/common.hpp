@@ -1,0 +1,212 @@
+// Shared device/host utilities for the gfx950 kernels of the SM preconditioner and PCG.
+//
+// Layout conventions used by every kernel:
+//   grid (n1, n2, n3), last axis fastest, V = n1*n2*n3, index v = (i*n2 + j)*n3 + k;
+//   batched vectors are [B][V]; images are [C][V]; stencil weights K are [27][V]
+//   (structure of arrays so that a wavefront streams 64 consecutive voxels of one tap);
+//   tap p = 9*(d1+1) + 3*(d2+1) + (d3+1) for the offset d in {-1,0,1}^3;
+//   values outside the grid are zero.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace mlpcg {
+
+typedef __bf16 bf16;
+typedef _Float16 f16;
+
+enum DType : int { kF64 = 0, kF32 = 1, kF16 = 2, kBF16 = 3 };
+
+// Accumulation type: FP64 storage accumulates in FP64, every other storage in FP32.
+template <typename T> struct AccOf { typedef float type; };
+template <> struct AccOf<double> { typedef double type; };
+
+constexpr int kThreads = 256;  // 4 wavefronts of 64 lanes
+
+// ---------------------------------------------------------------------------
+// Vector loads/stores of N consecutive elements, split into <=16-byte pieces so that
+// the only alignment requirement is min(16, N*sizeof(T)) bytes.
+// ---------------------------------------------------------------------------
+template <typename T, int N> struct Piece {
+  static constexpr int kBytes = (N * (int)sizeof(T) >= 16) ? 16 : N * (int)sizeof(T);
+  static constexpr int kElems = kBytes / (int)sizeof(T);
+};
+
+template <int N, typename A, typename T>
+__device__ __forceinline__ void load_vec(const T* __restrict__ p, A* out) {
+  constexpr int E = Piece<T, N>::kElems;
+  if constexpr (E >= 2) {
+    typedef T vt __attribute__((ext_vector_type(E)));
+#pragma unroll
+    for (int c = 0; c < N / E; ++c) {
+      vt v = *reinterpret_cast<const vt*>(p + c * E);
+#pragma unroll
+      for (int t = 0; t < E; ++t) out[c * E + t] = (A)v[t];
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < N; ++t) out[t] = (A)p[t];
+  }
+}
+
+template <int N, typename T, typename A>
+__device__ __forceinline__ void store_vec(T* __restrict__ p, const A* in) {
+  constexpr int E = Piece<T, N>::kElems;
+  if constexpr (E >= 2) {
+    typedef T vt __attribute__((ext_vector_type(E)));
+#pragma unroll
+    for (int c = 0; c < N / E; ++c) {
+      vt v;
+#pragma unroll
+      for (int t = 0; t < E; ++t) v[t] = (T)in[c * E + t];
+      *reinterpret_cast<vt*>(p + c * E) = v;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < N; ++t) p[t] = (T)in[t];
+  }
+}
+
+// Non-temporal variant for single-use streams (stencil weights): keeps the reused
+// vectors in L2 instead of the stream.
+template <int N, typename A, typename T>
+__device__ __forceinline__ void load_vec_nt(const T* __restrict__ p, A* out) {
+  constexpr int E = Piece<T, N>::kElems;
+  if constexpr (E >= 2) {
+    typedef T vt __attribute__((ext_vector_type(E)));
+#pragma unroll
+    for (int c = 0; c < N / E; ++c) {
+      vt v = __builtin_nontemporal_load(reinterpret_cast<const vt*>(p + c * E));
+#pragma unroll
+      for (int t = 0; t < E; ++t) out[c * E + t] = (A)v[t];
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < N; ++t) out[t] = (A)__builtin_nontemporal_load(p + t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware block order.  Consecutive workgroup ids are dealt round-robin over the
+// 8 XCDs; remapping gives every XCD a contiguous range of tiles so the halo rows that
+// neighbouring tiles share stay in one L2.
+// ---------------------------------------------------------------------------
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblk) {
+  const unsigned q = nblk / kNumXcd, r = nblk % kNumXcd;
+  const unsigned xcd = bid % kNumXcd, slot = bid / kNumXcd;
+  // XCDs [0, r) own q+1 tiles, the rest own q.
+  const unsigned base = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return base + slot;
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront / block reductions (64-lane wavefronts).
+// ---------------------------------------------------------------------------
+template <typename A>
+__device__ __forceinline__ A wave_sum(A v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over a 256-thread block; result valid in every thread.  `sh` holds >= 4 values.
+template <typename A>
+__device__ __forceinline__ A block_sum(A v, A* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  A r = sh[0] + sh[1] + sh[2] + sh[3];
+  return r;
+}
+
+// Deterministic grid reduction without global atomics on the data: every block writes
+// its NR partial sums, the last block to arrive (ticket counter) adds the partials in
+// block order and runs `fin` on the totals, then re-arms the counter for the next use.
+// partials: [NR][gridDim.x] doubles.  Must be called by all threads of the block.
+template <int NR, typename Fin>
+__device__ __forceinline__ void grid_reduce(double (&v)[NR], double* __restrict__ partials,
+                                            unsigned* __restrict__ counter, Fin fin) {
+  __shared__ double sh[NR][4];
+  __shared__ bool last;
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    double s = wave_sum(v[r]);
+    if ((threadIdx.x & 63) == 0) sh[r][w] = s;
+  }
+  __syncthreads();
+  const unsigned nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      partials[(size_t)r * nb + blockIdx.x] = sh[r][0] + sh[r][1] + sh[r][2] + sh[r][3];
+    __threadfence();
+    unsigned ticket = atomicAdd(counter, 1u);
+    last = (ticket == nb - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double tot[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    double s = 0.0;
+    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
+      s += __builtin_nontemporal_load(&partials[(size_t)r * nb + b]);
+    tot[r] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    double s = wave_sum(tot[r]);
+    if ((threadIdx.x & 63) == 0) sh[r][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) t[r] = sh[r][0] + sh[r][1] + sh[r][2] + sh[r][3];
+    fin(t);
+    *counter = 0u;
+    __threadfence();
+  }
+}
+
+// Early exit flag shared by all kernels of a running solve: once the solve has
+// converged the queued kernels of a speculative iteration return immediately.
+__device__ __forceinline__ bool solve_done(const int* done) {
+  return done != nullptr && __builtin_nontemporal_load(done) != 0;
+}
+
+}  // namespace mlpcg
+
+// ---------------------------------------------------------------------------
+// Host-side error plumbing for the C ABI.
+// ---------------------------------------------------------------------------
+namespace mlpcg {
+void set_error(const char* fmt, ...);
+int check_launch(const char* what);
+}  // namespace mlpcg
+
+#define MLPCG_REQUIRE(cond, ...)            \
+  do {                                      \
+    if (!(cond)) {                          \
+      ::mlpcg::set_error(__VA_ARGS__);      \
+      return -1;                            \
+    }                                       \
+  } while (0)
+
+#define MLPCG_HIP(call)                                                          \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      ::mlpcg::set_error("%s failed: %s", #call, hipGetErrorString(e_));         \
+      return -1;                                                                 \
+    }                                                                            \
+  } while (0)

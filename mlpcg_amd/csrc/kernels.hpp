@@ -1,0 +1,74 @@
+// Kernel argument blocks and host launchers shared by the C ABI and the native runtime.
+#pragma once
+
+#include "common.hpp"
+
+namespace mlpcg {
+
+// y[b] = epilogue( sum_p K[p] * x[b](. + d_p) ).  Epilogue, in order:
+//   val *= *in_scale              (if in_scale; FP64 scalar on the device)
+//   val = 0 where mask_a == 0     (if mask_a; the model's fluid mask)
+//   val = c0 * prev + c1 * val    (if prev; c0/c1 are device scalars of the accumulate type)
+//   val = 0 where mask_b == 0     (if mask_b; restriction to the unknowns)
+struct ApplyArgs {
+  const void* K;
+  const void* x;
+  void* y;
+  const void* prev;
+  const void* c0;
+  const void* c1;
+  const uint8_t* mask_a;
+  const uint8_t* mask_b;
+  const double* in_scale;
+  const int* done;
+  int B, n1, n2, n3;
+};
+
+// out[b] = scale * up(in[b])  (or c0 * prev + c1 * scale * up(in)); fine dims n1..n3.
+struct UpArgs {
+  const void* in;
+  void* out;
+  const void* prev;
+  const void* c0;
+  const void* c1;
+  const int* done;
+  double scale;
+  int nearest;
+  int B, n1, n2, n3;
+};
+
+int launch_stencil_weights(int dtype, const void* img, const void* W, const void* b, void* K, int C,
+                           int n1, int n2, int n3, bool trans, hipStream_t st);
+int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& a, hipStream_t st);
+int launch_avgpool2(int dtype, const void* in, void* out, int B, int n1, int n2, int n3,
+                    const int* done, hipStream_t st);
+int launch_upsample2(int dtype, const UpArgs& a, hipStream_t st);
+int launch_upsample2_adjoint(int dtype, const void* gin, void* gout, int B, int n1, int n2, int n3,
+                             int nearest, double scale, hipStream_t st);
+
+// SM linear: ybar[s,e] = mean_v img[s, v+e]; z = w . ybar + b.  ybar/z are of the
+// accumulate type; partials needs mean_patch_partials_size() bytes.
+size_t mean_patch_workspace(int dtype, int C, int n1, int n2, int n3);
+int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,
+                     void* workspace, int C, int n1, int n2, int n3, hipStream_t st);
+
+// Weight gradient of the SM block: gW[p,s,e] = sum_{b,v} a[b,v] c[b,v+d_p] img[s,v+e],
+// gb[p] = sum_{b,v} a[b,v] c[b,v+d_p]; (a, c) = (grad_out, x) for the block and
+// (x, grad_out) for the transpose.
+size_t weight_grad_workspace(int dtype, int C, int B, int n1, int n2, int n3);
+int launch_weight_grad(int dtype, const void* a, const void* c, const void* img, void* gW, void* gb,
+                       void* workspace, int C, int B, int n1, int n2, int n3, hipStream_t st);
+
+int launch_linear_backward(int dtype, const void* g, const void* ybar, void* gw, void* gb, int C,
+                           hipStream_t st);
+
+// Stencil weights with an image of another type (image in the accumulate type, K in
+// the storage type): tin in {F32, F64}.
+int launch_stencil_weights2(int tin, int tk, const void* img, const void* W, const void* b, void* K,
+                            int C, int n1, int n2, int n3, bool trans, hipStream_t st);
+
+int launch_nonzero_mask(int acc, const void* ch, uint8_t* m, long V, hipStream_t st);
+int launch_square(int acc, const void* in, void* out, int n, hipStream_t st);
+int launch_cast(int tin, int tout, const void* in, void* out, long n, hipStream_t st);
+
+}  // namespace mlpcg

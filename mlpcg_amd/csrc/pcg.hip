@@ -1,0 +1,578 @@
+// PCG kernels: the fluid Laplacian applied matrix-free from a per-voxel code (full grid)
+// or from CSR (compressed unknowns), fused with the vector updates and FP64 dot products
+// of cg_tests.py (npcg :26-101, npcg_flex :103-183, npsd :189-284).
+//
+// Full-grid ("flat") vectors are zero outside the unknowns.  code[v] = 0 for a non-fluid
+// cell, else 1 + (number of non-solid face neighbours) = 1 + A_vv; a neighbour outside
+// the grid counts as solid.  Then for a fluid v:  (A x)(v) = A_vv x(v) - sum_nbr x(nbr),
+// the sum running over all in-grid face neighbours (x vanishes on non-fluid cells).
+//
+// Every reduction is the deterministic last-block pattern of common.hpp; its finalizer
+// updates the solver scalars in SolveState on the device, so an iteration needs no host
+// round trip.  Every kernel returns at once when state->done is set.
+#include "common.hpp"
+#include "pcg.hpp"
+
+namespace mlpcg {
+
+constexpr int kMaxRedBlocks = 2048;
+
+__device__ __forceinline__ int grid_blocks_needed() { return gridDim.x; }
+
+// ---------------------------------------------------------------------------
+// code from flags
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void flags_code_kernel(const int8_t* __restrict__ flags,
+                                                              uint8_t* __restrict__ code, int n1,
+                                                              int n2, int n3) {
+  const long V = (long)n1 * n2 * n3;
+  const long v = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (v >= V) return;
+  const int k = (int)(v % n3), j = (int)((v / n3) % n2), i = (int)(v / ((long)n2 * n3));
+  if (flags[v] != kFlagFluid) {
+    code[v] = 0;
+    return;
+  }
+  const long s1 = (long)n2 * n3;
+  int d = 0;
+  if (i > 0 && flags[v - s1] >= kFlagFluid) ++d;
+  if (i < n1 - 1 && flags[v + s1] >= kFlagFluid) ++d;
+  if (j > 0 && flags[v - n3] >= kFlagFluid) ++d;
+  if (j < n2 - 1 && flags[v + n3] >= kFlagFluid) ++d;
+  if (k > 0 && flags[v - 1] >= kFlagFluid) ++d;
+  if (k < n3 - 1 && flags[v + 1] >= kFlagFluid) ++d;
+  code[v] = (uint8_t)(1 + d);
+}
+
+// ---------------------------------------------------------------------------
+// Flat stencil helpers: a thread owns a z-run of VZ voxels.
+// ---------------------------------------------------------------------------
+template <int VZ>
+struct Run {
+  int i, j, k0;
+  long v0;
+};
+
+template <int VZ>
+__device__ __forceinline__ bool run_at(long cid, int n1, int n2, int n3, Run<VZ>& r) {
+  const int n3c = n3 / VZ;
+  const long nch = (long)n1 * n2 * n3c;
+  if (cid >= nch) return false;
+  r.k0 = (int)(cid % n3c) * VZ;
+  const long t = cid / n3c;
+  r.j = (int)(t % n2);
+  r.i = (int)(t / n2);
+  r.v0 = ((long)r.i * n2 + r.j) * n3 + r.k0;
+  return true;
+}
+
+// Loads u = x + a*p (or just x when p == nullptr) on the 7-point neighbourhood of a run:
+// c[0..VZ+1] is the centre row with one halo cell each side, s[4][VZ] the four side rows
+// (i-1, i+1, j-1, j+1).  Out-of-grid values are 0.  u is rounded to FP32 (the storage
+// precision of x) so a residual computed from it is the residual of the stored iterate.
+template <typename TA>
+__device__ __forceinline__ TA upd(float x, float p, TA a, bool has_p) {
+  return has_p ? (TA)(float)((TA)x + a * (TA)p) : (TA)x;
+}
+
+template <int VZ, typename TA>
+__device__ __forceinline__ void load_star(const float* __restrict__ x, const float* __restrict__ p,
+                                          TA a, const Run<VZ>& r, int n1, int n2, int n3,
+                                          TA (&c)[VZ + 2], TA (&s)[4][VZ]) {
+  const long s1 = (long)n2 * n3;
+  float xv[VZ], pv[VZ];
+  load_vec<VZ>(x + r.v0, xv);
+  if (p) load_vec<VZ>(p + r.v0, pv);
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) c[t + 1] = upd<TA>(xv[t], p ? pv[t] : 0.f, a, p != nullptr);
+  c[0] = TA(0);
+  c[VZ + 1] = TA(0);
+  if (r.k0 > 0) c[0] = upd<TA>(x[r.v0 - 1], p ? p[r.v0 - 1] : 0.f, a, p != nullptr);
+  if (r.k0 + VZ < n3) c[VZ + 1] = upd<TA>(x[r.v0 + VZ], p ? p[r.v0 + VZ] : 0.f, a, p != nullptr);
+  const long offs[4] = {-s1, s1, -(long)n3, (long)n3};
+  const bool ok[4] = {r.i > 0, r.i < n1 - 1, r.j > 0, r.j < n2 - 1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (ok[q]) {
+      load_vec<VZ>(x + r.v0 + offs[q], xv);
+      if (p) load_vec<VZ>(p + r.v0 + offs[q], pv);
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) s[q][t] = upd<TA>(xv[t], p ? pv[t] : 0.f, a, p != nullptr);
+    } else {
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) s[q][t] = TA(0);
+    }
+  }
+}
+
+template <int VZ, typename TA>
+__device__ __forceinline__ TA lap_at(const TA (&c)[VZ + 2], const TA (&s)[4][VZ], int t, int code) {
+  const TA nb = c[t] + c[t + 2] + s[0][t] + s[1][t] + s[2][t] + s[3][t];
+  return (TA)(code - 1) * c[t + 1] - nb;
+}
+
+// r = b - A x_new,  x_new = x_old + alpha p (alpha from the state unless init).
+// Reduces [r.r, b.b].
+template <int VZ>
+__global__ __launch_bounds__(kThreads) void resid_flat_kernel(FlatArgs a, int init) {
+  SolveState* st = a.st;
+  if (!init && st->done) return;
+  const double alpha = init ? 0.0 : st->alpha;
+  const float* pp = init ? nullptr : a.p;
+  double red[2] = {0.0, 0.0};
+  const long nch = (long)a.n1 * a.n2 * (a.n3 / VZ);
+  for (long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; cid < nch;
+       cid += (long)gridDim.x * kThreads) {
+    Run<VZ> r;
+    run_at<VZ>(cid, a.n1, a.n2, a.n3, r);
+    double c[VZ + 2], s[4][VZ];
+    load_star<VZ, double>(a.x_old, pp, alpha, r, a.n1, a.n2, a.n3, c, s);
+    float bv[VZ], code[VZ];
+    load_vec<VZ>(a.b + r.v0, bv);
+    load_vec<VZ>(a.code + r.v0, code);
+    float rv[VZ], xn[VZ];
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) {
+      const int cd = (int)code[t];
+      const double res = cd ? (double)bv[t] - lap_at<VZ, double>(c, s, t, cd) : 0.0;
+      rv[t] = (float)res;
+      xn[t] = cd ? (float)c[t + 1] : 0.f;
+      red[0] += res * res;
+      red[1] += (double)bv[t] * (double)bv[t];
+    }
+    store_vec<VZ>(a.r + r.v0, rv);
+    if (!init) store_vec<VZ>(a.x_new + r.v0, xn);
+  }
+  grid_reduce<2>(red, a.partials, a.counter, [&](const double* t) {
+    if (init) {
+      st->norm_b = sqrt(t[1]);
+      st->iter = 0;
+      st->done = (t[0] == 0.0) ? 1 : 0;
+    } else {
+      st->iter += 1;
+      st->rho_old = st->rho;
+    }
+    st->rr = t[0];
+    st->norm = sqrt(t[0]) / st->norm_b;
+    st->scale = t[0] > 0.0 ? 1.0 / sqrt(t[0]) : 0.0;
+    if (!init) {
+      const double tol = fmax(st->tol, st->atol / st->norm_b);
+      const bool cont = (st->method == kNpsd) ? !(st->norm < tol) : (st->norm > tol);
+      st->done = (cont && st->iter < st->max_it) ? 0 : 1;
+    }
+    if (st->history && st->iter <= st->max_it) st->history[st->iter] = st->norm;
+  });
+}
+
+// p_new = z + beta p_old; w = A p_new; reduces p_new . w -> alpha = rho / (p.w).
+template <int VZ>
+__global__ __launch_bounds__(kThreads) void p_spmv_flat_kernel(FlatArgs a) {
+  SolveState* st = a.st;
+  if (st->done) return;
+  const float beta = (float)st->beta;
+  double red[1] = {0.0};
+  const long nch = (long)a.n1 * a.n2 * (a.n3 / VZ);
+  for (long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; cid < nch;
+       cid += (long)gridDim.x * kThreads) {
+    Run<VZ> r;
+    run_at<VZ>(cid, a.n1, a.n2, a.n3, r);
+    float c[VZ + 2], s[4][VZ];
+    // p_new = z + beta * p_old, evaluated on the neighbourhood
+    load_star<VZ, float>(a.z, a.p_old, beta, r, a.n1, a.n2, a.n3, c, s);
+    float code[VZ];
+    load_vec<VZ>(a.code + r.v0, code);
+    float wv[VZ], pn[VZ];
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) {
+      const int cd = (int)code[t];
+      const float w = cd ? lap_at<VZ, float>(c, s, t, cd) : 0.f;
+      pn[t] = cd ? c[t + 1] : 0.f;
+      wv[t] = w;
+      red[0] += (double)pn[t] * (double)w;
+    }
+    store_vec<VZ>(a.p_new + r.v0, pn);
+    store_vec<VZ>(a.w + r.v0, wv);
+  }
+  grid_reduce<1>(red, a.partials, a.counter, [&](const double* t) {
+    st->pw = t[0];
+    st->alpha = st->rho / t[0];
+  });
+}
+
+// rho = r.z (and z.(r - r_old) for the flexible variant) -> beta.
+__global__ __launch_bounds__(kThreads) void rz_dot_kernel(const float* __restrict__ r,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ r_old, long n,
+                                                          SolveState* st, double* partials,
+                                                          unsigned* counter) {
+  if (st->done) return;
+  double red[2] = {0.0, 0.0};
+  const long n4 = n / 4;
+  const bool flex = (st->method == kNpcgFlex) && r_old != nullptr;
+  for (long q = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; q < n4 + (n & 3);
+       q += (long)gridDim.x * kThreads) {
+    if (q < n4) {
+      float rv[4], zv[4];
+      load_vec<4>(r + 4 * q, rv);
+      load_vec<4>(z + 4 * q, zv);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) red[0] += (double)rv[t] * (double)zv[t];
+      if (flex) {
+        float ov[4];
+        load_vec<4>(r_old + 4 * q, ov);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) red[1] += (double)zv[t] * ((double)rv[t] - (double)ov[t]);
+      }
+    } else {
+      const long e = 4 * n4 + (q - n4);
+      red[0] += (double)r[e] * (double)z[e];
+      if (flex) red[1] += (double)z[e] * ((double)r[e] - (double)r_old[e]);
+    }
+  }
+  grid_reduce<2>(red, partials, counter, [&](const double* t) {
+    st->rho = t[0];
+    if (st->iter == 0) st->beta = 0.0;
+    else st->beta = (flex ? t[1] : t[0]) / st->rho_old;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// CSR (compressed unknowns)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void scatter_kernel(const int* __restrict__ idx,
+                                                           const float* __restrict__ src,
+                                                           float* __restrict__ grid, long n,
+                                                           const int* done) {
+  if (solve_done(done)) return;
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) grid[idx[i]] = src[i];
+}
+
+__global__ __launch_bounds__(kThreads) void gather_kernel(const int* __restrict__ idx,
+                                                          const float* __restrict__ grid,
+                                                          float* __restrict__ dst, long n,
+                                                          const int* done) {
+  if (solve_done(done)) return;
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) dst[i] = grid[idx[i]];
+}
+
+__global__ __launch_bounds__(kThreads) void p_spmv_csr_kernel(CsrArgs a) {
+  SolveState* st = a.st;
+  if (st->done) return;
+  const float beta = (float)st->beta;
+  double red[1] = {0.0};
+  for (long row = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; row < a.n;
+       row += (long)gridDim.x * kThreads) {
+    const int e0 = a.rowptr[row], e1 = a.rowptr[row + 1];
+    float w = 0.f;
+    for (int e = e0; e < e1; ++e) {
+      const int col = a.col[e];
+      w += a.val[e] * (a.z[col] + beta * a.p_old[col]);
+    }
+    const float pn = a.z[row] + beta * a.p_old[row];
+    a.p_new[row] = pn;
+    a.w[row] = w;
+    red[0] += (double)pn * (double)w;
+  }
+  grid_reduce<1>(red, a.partials, a.counter, [&](const double* t) {
+    st->pw = t[0];
+    st->alpha = st->rho / t[0];
+  });
+}
+
+__global__ __launch_bounds__(kThreads) void resid_csr_kernel(CsrArgs a, int init) {
+  SolveState* st = a.st;
+  if (!init && st->done) return;
+  const double alpha = init ? 0.0 : st->alpha;
+  double red[2] = {0.0, 0.0};
+  for (long row = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; row < a.n;
+       row += (long)gridDim.x * kThreads) {
+    const int e0 = a.rowptr[row], e1 = a.rowptr[row + 1];
+    double ax = 0.0;
+    for (int e = e0; e < e1; ++e) {
+      const int col = a.col[e];
+      const double xc = init ? (double)a.x_old[col]
+                            : (double)(float)((double)a.x_old[col] + alpha * (double)a.p[col]);
+      ax += (double)a.val[e] * xc;
+    }
+    const double bv = a.b[row];
+    const double res = bv - ax;
+    a.r[row] = (float)res;
+    if (!init) a.x_new[row] = (float)((double)a.x_old[row] + alpha * (double)a.p[row]);
+    red[0] += res * res;
+    red[1] += bv * bv;
+  }
+  grid_reduce<2>(red, a.partials, a.counter, [&](const double* t) {
+    if (init) {
+      st->norm_b = sqrt(t[1]);
+      st->iter = 0;
+      st->done = (t[0] == 0.0) ? 1 : 0;
+    } else {
+      st->iter += 1;
+      st->rho_old = st->rho;
+    }
+    st->rr = t[0];
+    st->norm = sqrt(t[0]) / st->norm_b;
+    st->scale = t[0] > 0.0 ? 1.0 / sqrt(t[0]) : 0.0;
+    if (!init) {
+      const double tol = fmax(st->tol, st->atol / st->norm_b);
+      const bool cont = (st->method == kNpsd) ? !(st->norm < tol) : (st->norm > tol);
+      st->done = (cont && st->iter < st->max_it) ? 0 : 1;
+    }
+    if (st->history && st->iter <= st->max_it) st->history[st->iter] = st->norm;
+  });
+}
+
+// Plain y = A x (CSR, FP32) and y = A x (flat, FP32) for the drop-in operator.
+__global__ __launch_bounds__(kThreads) void spmv_csr_kernel(const int* __restrict__ rowptr,
+                                                            const int* __restrict__ col,
+                                                            const float* __restrict__ val,
+                                                            const float* __restrict__ x,
+                                                            float* __restrict__ y, long n) {
+  const long row = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (row >= n) return;
+  float s = 0.f;
+  for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) s += val[e] * x[col[e]];
+  y[row] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void spmv_csr_f64_kernel(const int* __restrict__ rowptr,
+                                                                const int* __restrict__ col,
+                                                                const double* __restrict__ val,
+                                                                const double* __restrict__ x,
+                                                                double* __restrict__ y, long n) {
+  const long row = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (row >= n) return;
+  double s = 0.0;
+  for (int e = rowptr[row]; e < rowptr[row + 1]; ++e) s += val[e] * x[col[e]];
+  y[row] = s;
+}
+
+template <int VZ>
+__global__ __launch_bounds__(kThreads) void spmv_flat_kernel(const uint8_t* __restrict__ code,
+                                                             const float* __restrict__ x,
+                                                             float* __restrict__ y, int n1, int n2,
+                                                             int n3) {
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  Run<VZ> r;
+  if (!run_at<VZ>(cid, n1, n2, n3, r)) return;
+  float c[VZ + 2], s[4][VZ];
+  load_star<VZ, float>(x, nullptr, 0.f, r, n1, n2, n3, c, s);
+  float cd[VZ], out[VZ];
+  load_vec<VZ>(code + r.v0, cd);
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) out[t] = cd[t] != 0.f ? lap_at<VZ, float>(c, s, t, (int)cd[t]) : 0.f;
+  store_vec<VZ>(y + r.v0, out);
+}
+
+// sum_i a[i] b[i] in FP64 (drop-in dot for torch-free callers).
+__global__ __launch_bounds__(kThreads) void dot_kernel(const float* __restrict__ x,
+                                                       const float* __restrict__ y, long n,
+                                                       double* partials, unsigned* counter,
+                                                       double* out) {
+  double red[1] = {0.0};
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
+    red[0] += (double)x[i] * (double)y[i];
+  grid_reduce<1>(red, partials, counter, [&](const double* t) { out[0] = t[0]; });
+}
+
+// ---------------------------------------------------------------------------
+// npsd (cg_tests.py:217-281): projections against the last directions, then the
+// A-product and the coefficient q.r / q.Aq.
+// ---------------------------------------------------------------------------
+// coef = (q . Ap_j) / alfa_j; stored in st->proj.
+__global__ __launch_bounds__(kThreads) void npsd_proj_dot_kernel(const float* __restrict__ q,
+                                                                 const float* __restrict__ Ap,
+                                                                 const double* __restrict__ alfa_j,
+                                                                 long n, SolveState* st,
+                                                                 double* partials,
+                                                                 unsigned* counter) {
+  if (st->done) return;
+  double red[1] = {0.0};
+  for (long i = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; i < n;
+       i += (long)gridDim.x * kThreads)
+    red[0] += (double)q[i] * (double)Ap[i];
+  grid_reduce<1>(red, partials, counter, [&](const double* t) { st->proj = t[0] / alfa_j[0]; });
+}
+
+// q -= proj * p_j
+__global__ __launch_bounds__(kThreads) void npsd_axpy_kernel(float* __restrict__ q,
+                                                             const float* __restrict__ pj, long n,
+                                                             const SolveState* st) {
+  if (st->done) return;
+  const float c = (float)st->proj;
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
+    q[i] = q[i] - c * pj[i];
+}
+
+// Aq = A q (flat), alfa = q.Aq, beta = q.r / alfa; beta goes to st->alpha so the
+// residual kernel applies x += beta q.
+template <int VZ>
+__global__ __launch_bounds__(kThreads) void npsd_spmv_flat_kernel(FlatArgs a, double* alfa_out) {
+  SolveState* st = a.st;
+  if (st->done) return;
+  double red[2] = {0.0, 0.0};
+  const long nch = (long)a.n1 * a.n2 * (a.n3 / VZ);
+  for (long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; cid < nch;
+       cid += (long)gridDim.x * kThreads) {
+    Run<VZ> r;
+    run_at<VZ>(cid, a.n1, a.n2, a.n3, r);
+    float c[VZ + 2], s[4][VZ];
+    load_star<VZ, float>(a.z, nullptr, 0.f, r, a.n1, a.n2, a.n3, c, s);
+    float code[VZ], rv[VZ], wv[VZ];
+    load_vec<VZ>(a.code + r.v0, code);
+    load_vec<VZ>(a.r + r.v0, rv);
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) {
+      const int cd = (int)code[t];
+      wv[t] = cd ? lap_at<VZ, float>(c, s, t, cd) : 0.f;
+      red[0] += (double)c[t + 1] * (double)wv[t];
+      red[1] += (double)c[t + 1] * (double)rv[t];
+    }
+    store_vec<VZ>(a.w + r.v0, wv);
+  }
+  grid_reduce<2>(red, a.partials, a.counter, [&](const double* t) {
+    alfa_out[0] = t[0];
+    st->alpha = t[1] / t[0];
+  });
+}
+
+__global__ __launch_bounds__(kThreads) void npsd_spmv_csr_kernel(CsrArgs a, double* alfa_out) {
+  SolveState* st = a.st;
+  if (st->done) return;
+  double red[2] = {0.0, 0.0};
+  for (long row = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; row < a.n;
+       row += (long)gridDim.x * kThreads) {
+    float w = 0.f;
+    for (int e = a.rowptr[row]; e < a.rowptr[row + 1]; ++e) w += a.val[e] * a.z[a.col[e]];
+    a.w[row] = w;
+    red[0] += (double)a.z[row] * (double)w;
+    red[1] += (double)a.z[row] * (double)a.r[row];
+  }
+  grid_reduce<2>(red, a.partials, a.counter, [&](const double* t) {
+    alfa_out[0] = t[0];
+    st->alpha = t[1] / t[0];
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static int flat_vz(int n3) { return (n3 % 8 == 0) ? 8 : (n3 % 4 == 0 ? 4 : (n3 % 2 == 0 ? 2 : 1)); }
+
+static unsigned red_blocks(long items) {
+  long nb = (items + kThreads - 1) / kThreads;
+  if (nb > kMaxRedBlocks) nb = kMaxRedBlocks;
+  return (unsigned)(nb < 1 ? 1 : nb);
+}
+
+size_t reduction_workspace_bytes() { return (size_t)kMaxRedBlocks * 4 * sizeof(double) + 256; }
+
+int launch_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3, hipStream_t st) {
+  const long V = (long)n1 * n2 * n3;
+  flags_code_kernel<<<(unsigned)((V + kThreads - 1) / kThreads), kThreads, 0, st>>>(flags, code, n1,
+                                                                                     n2, n3);
+  return check_launch("flags_code");
+}
+
+#define MLPCG_FLAT_DISPATCH(KERN, ITEMS_VZ, ...)                                                \
+  do {                                                                                          \
+    const int vz_ = flat_vz(a.n3);                                                              \
+    const unsigned nb_ = red_blocks((long)a.n1 * a.n2 * (a.n3 / vz_));                          \
+    if (vz_ == 8) KERN<8><<<nb_, kThreads, 0, st>>>(__VA_ARGS__);                               \
+    else if (vz_ == 4) KERN<4><<<nb_, kThreads, 0, st>>>(__VA_ARGS__);                          \
+    else if (vz_ == 2) KERN<2><<<nb_, kThreads, 0, st>>>(__VA_ARGS__);                          \
+    else KERN<1><<<nb_, kThreads, 0, st>>>(__VA_ARGS__);                                        \
+  } while (0)
+
+int launch_resid_flat(const FlatArgs& a, int init, hipStream_t st) {
+  MLPCG_FLAT_DISPATCH(resid_flat_kernel, 0, a, init);
+  return check_launch("resid_flat");
+}
+
+int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st) {
+  MLPCG_FLAT_DISPATCH(p_spmv_flat_kernel, 0, a);
+  return check_launch("p_spmv_flat");
+}
+
+int launch_npsd_spmv_flat(const FlatArgs& a, double* alfa_out, hipStream_t st) {
+  MLPCG_FLAT_DISPATCH(npsd_spmv_flat_kernel, 0, a, alfa_out);
+  return check_launch("npsd_spmv_flat");
+}
+
+int launch_rz_dot(const float* r, const float* z, const float* r_old, long n, SolveState* s,
+                  double* partials, unsigned* counter, hipStream_t st) {
+  rz_dot_kernel<<<red_blocks(n / 4 + 1), kThreads, 0, st>>>(r, z, r_old, n, s, partials, counter);
+  return check_launch("rz_dot");
+}
+
+int launch_scatter(const int* idx, const float* src, float* grid, long n, const int* done,
+                   hipStream_t st) {
+  scatter_kernel<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(idx, src, grid, n, done);
+  return check_launch("scatter");
+}
+
+int launch_gather(const int* idx, const float* grid, float* dst, long n, const int* done,
+                  hipStream_t st) {
+  gather_kernel<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(idx, grid, dst, n, done);
+  return check_launch("gather");
+}
+
+int launch_p_spmv_csr(const CsrArgs& a, hipStream_t st) {
+  p_spmv_csr_kernel<<<red_blocks(a.n), kThreads, 0, st>>>(a);
+  return check_launch("p_spmv_csr");
+}
+
+int launch_resid_csr(const CsrArgs& a, int init, hipStream_t st) {
+  resid_csr_kernel<<<red_blocks(a.n), kThreads, 0, st>>>(a, init);
+  return check_launch("resid_csr");
+}
+
+int launch_npsd_spmv_csr(const CsrArgs& a, double* alfa_out, hipStream_t st) {
+  npsd_spmv_csr_kernel<<<red_blocks(a.n), kThreads, 0, st>>>(a, alfa_out);
+  return check_launch("npsd_spmv_csr");
+}
+
+int launch_npsd_proj(const float* q, const float* Ap, const double* alfa_j, long n, SolveState* s,
+                     double* partials, unsigned* counter, hipStream_t st) {
+  npsd_proj_dot_kernel<<<red_blocks(n), kThreads, 0, st>>>(q, Ap, alfa_j, n, s, partials, counter);
+  return check_launch("npsd_proj");
+}
+
+int launch_npsd_axpy(float* q, const float* pj, long n, const SolveState* s, hipStream_t st) {
+  npsd_axpy_kernel<<<red_blocks(n), kThreads, 0, st>>>(q, pj, n, s);
+  return check_launch("npsd_axpy");
+}
+
+int launch_spmv_csr(const int* rowptr, const int* col, const float* val, const float* x, float* y,
+                    long n, hipStream_t st) {
+  spmv_csr_kernel<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(rowptr, col, val, x, y, n);
+  return check_launch("spmv_csr");
+}
+
+int launch_spmv_csr_f64(const int* rowptr, const int* col, const double* val, const double* x,
+                        double* y, long n, hipStream_t st) {
+  spmv_csr_f64_kernel<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, st>>>(rowptr, col, val, x, y, n);
+  return check_launch("spmv_csr_f64");
+}
+
+int launch_spmv_flat(const uint8_t* code, const float* x, float* y, int n1, int n2, int n3,
+                     hipStream_t st) {
+  const int vz = flat_vz(n3);
+  const long items = (long)n1 * n2 * (n3 / vz);
+  const unsigned nb = (unsigned)((items + kThreads - 1) / kThreads);
+  if (vz == 8) spmv_flat_kernel<8><<<nb, kThreads, 0, st>>>(code, x, y, n1, n2, n3);
+  else if (vz == 4) spmv_flat_kernel<4><<<nb, kThreads, 0, st>>>(code, x, y, n1, n2, n3);
+  else if (vz == 2) spmv_flat_kernel<2><<<nb, kThreads, 0, st>>>(code, x, y, n1, n2, n3);
+  else spmv_flat_kernel<1><<<nb, kThreads, 0, st>>>(code, x, y, n1, n2, n3);
+  return check_launch("spmv_flat");
+}
+
+int launch_dot(const float* x, const float* y, long n, double* partials, unsigned* counter,
+               double* out, hipStream_t st) {
+  dot_kernel<<<red_blocks(n), kThreads, 0, st>>>(x, y, n, partials, counter, out);
+  return check_launch("dot");
+}
+
+}  // namespace mlpcg

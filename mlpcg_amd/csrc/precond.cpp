@@ -1,0 +1,281 @@
+// Preconditioner plan: setup (pyramid, stencils, coefficients) and apply.
+//
+// Block order in Precond::K and in the weights/biases arrays handed to precond_create:
+//   kSmallSM  (SmallSMModelDn3D, sm_model_3d.py:164-287):
+//             pre[0..n-1], post[0..n-1], l            then c0[0..n-1], c1[0..n-1]
+//   kSPD      (SPDSMModelDn3D, :293-383):
+//             l0[0..n-1], l1[0..n-1], l               then c0[0..n-1], c1[0..n-1]
+//             (K also holds the transposed stencils l1_t[0..n-1], l0_t[0..n-1], l_t)
+//   kSmallSPD (SmallSPDSMModelDn3D, :388-444):
+//             l0[0..n-1], l                           then c0[0..n-1], c1[0..n-1]
+//             (K also holds l0_t[0..n-1], l_t)
+#include <cstring>
+
+#include "runtime.hpp"
+
+namespace mlpcg {
+
+static size_t dsize(int dt) {
+  switch (dt) {
+    case kF64: return 8;
+    case kF32: return 4;
+    default: return 2;
+  }
+}
+
+Precond::~Precond() {
+  for (void* p : allocs) (void)hipFree(p);
+}
+
+static int palloc(Precond* P, void** out, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  MLPCG_HIP(hipMalloc(out, bytes));
+  P->allocs.push_back(*out);
+  P->bytes += bytes;
+  return 0;
+}
+
+int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3, int dtype,
+                   int nearest, int mask, int swap, const void* image, const void* const* weights,
+                   const void* const* biases, hipStream_t st) {
+  MLPCG_REQUIRE(out != nullptr, "precond_create: null output handle");
+  MLPCG_REQUIRE(kind >= kSmallSM && kind <= kSmallSPD, "precond_create: bad model kind %d", kind);
+  MLPCG_REQUIRE(n >= 1 && n < kMaxLevels, "precond_create: levels must be in [1,%d)", kMaxLevels);
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "precond_create: C must be 1, 2 or 3");
+  MLPCG_REQUIRE(dtype >= kF64 && dtype <= kBF16, "precond_create: bad dtype %d", dtype);
+  const int div = 1 << n;
+  MLPCG_REQUIRE(n1 % div == 0 && n2 % div == 0 && n3 % div == 0,
+                "precond_create: grid (%d,%d,%d) must be divisible by 2^levels = %d", n1, n2, n3, div);
+  MLPCG_REQUIRE(!mask || C >= 2, "precond_create: mask needs the fluid image channel (C >= 2)");
+  MLPCG_REQUIRE(kind == kSmallSM || (!mask && !swap), "precond_create: mask/swap only for SmallSM");
+  Precond* P = new Precond();
+  P->kind = kind;
+  P->n = n;
+  P->C = C;
+  P->dtype = dtype;
+  P->acc = dtype == kF64 ? kF64 : kF32;
+  P->nearest = (kind == kSmallSM) ? nearest : 1;
+  P->mask = mask;
+  P->swap = swap;
+  const size_t es = dsize(dtype), as = dsize(P->acc);
+  int rc = 0;
+#define TRY(x)          \
+  do {                  \
+    rc = (x);           \
+    if (rc) {           \
+      delete P;         \
+      return rc;        \
+    }                   \
+  } while (0)
+  for (int l = 0; l <= n; ++l) {
+    P->dims[l][0] = n1 >> l;
+    P->dims[l][1] = n2 >> l;
+    P->dims[l][2] = n3 >> l;
+    P->V[l] = (long)P->dims[l][0] * P->dims[l][1] * P->dims[l][2];
+    TRY(palloc(P, &P->img[l], (size_t)C * P->V[l] * as));
+    TRY(palloc(P, &P->xd[l], (size_t)P->V[l] * es));
+    TRY(palloc(P, &P->xo[l], (size_t)P->V[l] * es));
+    TRY(palloc(P, &P->tmp[l], (size_t)P->V[l] * es));
+    if (kind == kSPD) TRY(palloc(P, &P->xs[l], (size_t)P->V[l] * es));
+  }
+  // image pyramid in the accumulate precision
+  MLPCG_HIP(hipMemcpyAsync(P->img[0], image, (size_t)C * P->V[0] * as, hipMemcpyDeviceToDevice, st));
+  for (int l = 1; l <= n; ++l)
+    TRY(launch_avgpool2(P->acc, P->img[l - 1], P->img[l], C, P->dims[l - 1][0], P->dims[l - 1][1],
+                        P->dims[l - 1][2], nullptr, st));
+  if (mask) {
+    void* m = nullptr;
+    TRY(palloc(P, &m, (size_t)P->V[0]));
+    P->maskimg = (uint8_t*)m;
+    TRY(launch_nonzero_mask(P->acc, (const char*)P->img[0] + (size_t)P->V[0] * as, P->maskimg, P->V[0], st));
+  }
+  // stencils: (weight index, level, transposed)
+  struct Blk { int w, level, trans; };
+  std::vector<Blk> blocks;
+  int nw = 0;
+  if (kind == kSmallSM) {
+    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 0});          // pre
+    for (int i = 0; i < n; ++i) blocks.push_back({n + i, i, 0});      // post
+    blocks.push_back({2 * n, n, 0});                                  // l
+    nw = 2 * n + 1;
+  } else if (kind == kSPD) {
+    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 0});          // l0
+    for (int i = 0; i < n; ++i) blocks.push_back({n + i, i, 0});      // l1
+    blocks.push_back({2 * n, n, 0});                                  // l
+    for (int i = 0; i < n; ++i) blocks.push_back({n + i, i, 1});      // l1_t
+    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 1});          // l0_t
+    blocks.push_back({2 * n, n, 1});                                  // l_t
+    nw = 2 * n + 1;
+  } else {
+    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 0});          // l0
+    blocks.push_back({n, n, 0});                                      // l
+    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 1});          // l0_t
+    blocks.push_back({n, n, 1});                                      // l_t
+    nw = n + 1;
+  }
+  for (const Blk& b : blocks) {
+    void* K = nullptr;
+    TRY(palloc(P, &K, (size_t)27 * P->V[b.level] * es));
+    P->K.push_back(K);
+    const int* d = P->dims[b.level];
+    TRY(launch_stencil_weights2(P->acc, dtype, P->img[b.level], weights[b.w], biases[b.w], K, C, d[0],
+                                d[1], d[2], b.trans != 0, st));
+  }
+  // SMLinear coefficients c0[i], c1[i] from img[i] (plus their squares for kSmallSPD)
+  TRY(palloc(P, &P->coef, (size_t)4 * n * as));
+  void* ws = nullptr;
+  TRY(palloc(P, &ws, mean_patch_workspace(P->acc, C, n1, n2, n3)));
+  void* ybar = nullptr;
+  TRY(palloc(P, &ybar, (size_t)27 * C * as));
+  for (int which = 0; which < 2; ++which) {
+    for (int i = 0; i < n; ++i) {
+      const int widx = nw + which * n + i;
+      char* z = (char*)P->coef + (size_t)(which * n + i) * as;
+      const int* d = P->dims[i];
+      TRY(launch_sm_linear(P->acc, P->img[i], weights[widx], biases[widx], z, ybar, ws, C, d[0], d[1],
+                           d[2], st));
+    }
+  }
+  if (kind == kSmallSPD) TRY(launch_square(P->acc, P->coef, (char*)P->coef + (size_t)2 * n * as, 2 * n, st));
+#undef TRY
+  *out = P;
+  return 0;
+}
+
+int Precond::apply(const float* r_in, float* z_out, const double* in_scale, const uint8_t* out_mask,
+                   const int* done, hipStream_t st) {
+  const int vdt = dtype == kF64 ? kF64 : kF32;  // type of the solver-side vectors
+  const size_t as = dsize(acc);
+  auto c0p = [&](int i, int sq) { return (const void*)((const char*)coef + (size_t)(sq * 2 * n + i) * as); };
+  auto c1p = [&](int i, int sq) { return (const void*)((const char*)coef + (size_t)(sq * 2 * n + n + i) * as); };
+  auto apply_blk = [&](int k, int level, const void* x, int tx, void* y, int ty, const void* prev,
+                       const void* c0, const void* c1, const uint8_t* ma, const uint8_t* mb,
+                       const double* sc) {
+    ApplyArgs a;
+    a.K = K[k];
+    a.x = x;
+    a.y = y;
+    a.prev = prev;
+    a.c0 = c0;
+    a.c1 = c1;
+    a.mask_a = ma;
+    a.mask_b = mb;
+    a.in_scale = sc;
+    a.done = done;
+    a.B = 1;
+    a.n1 = dims[level][0];
+    a.n2 = dims[level][1];
+    a.n3 = dims[level][2];
+    return launch_sm_apply(dtype, tx, ty, a, st);
+  };
+  auto pool = [&](const void* in, void* outp, int fine_level) {
+    const int* d = dims[fine_level];
+    return launch_avgpool2(dtype, in, outp, 1, d[0], d[1], d[2], done, st);
+  };
+  auto up = [&](const void* in, void* outp, int fine_level, double scale, const void* prev,
+                const void* c0, const void* c1) {
+    UpArgs u;
+    u.in = in;
+    u.out = outp;
+    u.prev = prev;
+    u.c0 = c0;
+    u.c1 = c1;
+    u.done = done;
+    u.scale = scale;
+    u.nearest = nearest;
+    u.B = 1;
+    u.n1 = dims[fine_level][0];
+    u.n2 = dims[fine_level][1];
+    u.n3 = dims[fine_level][2];
+    return launch_upsample2(dtype, u, st);
+  };
+  int rc;
+#define TRY(x)           \
+  do {                   \
+    rc = (x);            \
+    if (rc) return rc;   \
+  } while (0)
+
+  // ---- downward half (identical for every kind): x_0 = B0(r), x_i = Bi(pool x_{i-1}) ----
+  TRY(apply_blk(0, 0, r_in, vdt, xd[0], dtype, nullptr, nullptr, nullptr, mask ? maskimg : nullptr,
+                nullptr, in_scale));
+  for (int i = 1; i < n; ++i) {
+    TRY(pool(xd[i - 1], tmp[i], i - 1));
+    TRY(apply_blk(i, i, tmp[i], dtype, xd[i], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+  }
+  TRY(pool(xd[n - 1], tmp[n], n - 1));
+  const int kl = (kind == kSmallSPD) ? n : 2 * n;  // index of l
+  TRY(apply_blk(kl, n, tmp[n], dtype, xd[n], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+
+  if (kind == kSmallSM) {
+    const void* cur = xd[n];
+    for (int i = n; i >= 1; --i) {
+      const int lvl = i - 1;
+      const bool last = (lvl == 0);
+      void* y = last ? (void*)z_out : xo[lvl];
+      const int ty = last ? vdt : dtype;
+      const uint8_t* ma = (last && mask) ? maskimg : nullptr;
+      const uint8_t* mb = last ? out_mask : nullptr;
+      if (!swap) {
+        TRY(up(cur, tmp[lvl], lvl, 1.0, nullptr, nullptr, nullptr));
+        TRY(apply_blk(n + lvl, lvl, tmp[lvl], dtype, y, ty, xd[lvl], c0p(lvl, 0), c1p(lvl, 0), ma, mb, nullptr));
+      } else {
+        TRY(up(cur, tmp[lvl], lvl, 1.0, xd[lvl], c0p(lvl, 0), c1p(lvl, 0)));
+        TRY(apply_blk(n + lvl, lvl, tmp[lvl], dtype, y, ty, nullptr, nullptr, nullptr, ma, mb, nullptr));
+      }
+      cur = xo[lvl];
+    }
+    return 0;
+  }
+
+  if (kind == kSPD) {
+    // first half upward: x_i = l1[i-1](up(x_i)/8); x_{i-1} = c0 x_{i-1} + c1 x_i
+    const void* cur = xd[n];
+    for (int i = n; i >= 1; --i) {
+      const int lvl = i - 1;
+      TRY(up(cur, tmp[lvl], lvl, 0.125, nullptr, nullptr, nullptr));
+      TRY(apply_blk(n + lvl, lvl, tmp[lvl], dtype, xo[lvl], dtype, xd[lvl], c0p(lvl, 0), c1p(lvl, 0),
+                    nullptr, nullptr, nullptr));
+      cur = xo[lvl];
+    }
+    // second half downward: xs_0 = x_0 ; xs_{i+1} = pool(l1_t[i](xs_i)); xs_n = l_t(xs_n)
+    const int kt1 = 2 * n + 1, kt0 = 3 * n + 1, ktl = 4 * n + 1;
+    const void* xs0 = xo[0];
+    for (int i = 0; i < n; ++i) {
+      const void* src = (i == 0) ? xs0 : xs[i];
+      TRY(apply_blk(kt1 + i, i, src, dtype, tmp[i], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+      TRY(pool(tmp[i], xs[i + 1], i));
+    }
+    TRY(apply_blk(ktl, n, xs[n], dtype, xd[n], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+    // second half upward: xs_i = l0_t[i](c0 xs_i + c1 up(xs_{i+1})/8)
+    cur = xd[n];
+    for (int i = n - 1; i >= 0; --i) {
+      const void* prev = (i == 0) ? xs0 : xs[i];
+      TRY(up(cur, tmp[i], i, 0.125, prev, c0p(i, 0), c1p(i, 0)));
+      const bool last = (i == 0);
+      void* y = last ? (void*)z_out : xd[i];
+      TRY(apply_blk(kt0 + i, i, tmp[i], dtype, y, last ? vdt : dtype, nullptr, nullptr, nullptr, nullptr,
+                    last ? out_mask : nullptr, nullptr));
+      cur = xd[i];
+    }
+    return 0;
+  }
+
+  // kSmallSPD: x_n = l_t(x_n); x_{i-1} = l0_t[i-1](c0^2 x_{i-1} + c1^2 up(x_i)/8)
+  const int kt0 = n + 1, ktl = 2 * n + 1;
+  TRY(apply_blk(ktl, n, xd[n], dtype, xo[n], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+  const void* cur = xo[n];
+  for (int i = n; i >= 1; --i) {
+    const int lvl = i - 1;
+    TRY(up(cur, tmp[lvl], lvl, 0.125, xd[lvl], c0p(lvl, 1), c1p(lvl, 1)));
+    const bool last = (lvl == 0);
+    void* y = last ? (void*)z_out : xo[lvl];
+    TRY(apply_blk(kt0 + lvl, lvl, tmp[lvl], dtype, y, last ? vdt : dtype, nullptr, nullptr, nullptr, nullptr,
+                  last ? out_mask : nullptr, nullptr));
+    cur = xo[lvl];
+  }
+  return 0;
+#undef TRY
+}
+
+}  // namespace mlpcg

@@ -1,0 +1,350 @@
+// Reductions of the SM preconditioner: the SMLinear mean patch (forward) and the SM
+// block weight gradient (backward).  Both are deterministic two-pass reductions
+// (block partials in a workspace, then one ordered pass), with no global atomics.
+//
+//   mean patch  ybar[s,e] = (1/V) sum_v img[s, v+e]          sm_linear_3d_kernel.cu:62-83
+//               z = sum_{s,e} w[s,e] ybar[s,e] + b           sm_linear_3d_kernel.cu:185-186
+//   weight grad G_p(v) = sum_b a[b,v] c[b,v+d_p]              sm_block_3d_kernel.cu:121-229
+//               gW[p,s,e] = sum_v G_p(v) img[s,v+e], gb[p] = sum_v G_p(v)
+//               i.e. the [27 x V] . [V x (27C+1)] product of per-voxel outer products,
+//               tiled through LDS with a 3x4 register tile per thread.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mlpcg {
+
+// ---------------------------------------------------------------------------
+// Mean patch.  A voxel u contributes to ybar[s,e] iff u - e is inside the grid, i.e.
+// per axis: e=-1 -> u <= n-2, e=0 -> always, e=+1 -> u >= 1.  Each thread sums a
+// z-run of VZ voxels once and spreads it over the 27 offsets with those masks.
+// ---------------------------------------------------------------------------
+template <typename TI, int C, int VZ>
+__global__ __launch_bounds__(kThreads) void mean_patch_partial_kernel(const TI* __restrict__ img,
+                                                                      int n1, int n2, int n3,
+                                                                      double* __restrict__ partials) {
+  typedef typename AccOf<TI>::type A;
+  constexpr int NV = 27 * C;
+  A acc[NV];
+#pragma unroll
+  for (int o = 0; o < NV; ++o) acc[o] = A(0);
+  const long V = (long)n1 * n2 * n3;
+  const int n3c = n3 / VZ;
+  const long nch = (long)n1 * n2 * n3c;
+  for (long cid = (long)blockIdx.x * kThreads + threadIdx.x; cid < nch;
+       cid += (long)gridDim.x * kThreads) {
+    const int kc = (int)(cid % n3c);
+    const long t = cid / n3c;
+    const int j = (int)(t % n2), i = (int)(t / n2);
+    const int k0 = kc * VZ;
+    const long v0 = ((long)i * n2 + j) * n3 + k0;
+    const A m1[3] = {i <= n1 - 2 ? A(1) : A(0), A(1), i >= 1 ? A(1) : A(0)};
+    const A m2[3] = {j <= n2 - 2 ? A(1) : A(0), A(1), j >= 1 ? A(1) : A(0)};
+#pragma unroll
+    for (int s = 0; s < C; ++s) {
+      A v[VZ];
+      load_vec<VZ>(img + s * V + v0, v);
+      A cs = A(0);
+#pragma unroll
+      for (int u = 0; u < VZ; ++u) cs += v[u];
+      const A first = (k0 == 0) ? v[0] : A(0);
+      const A last = (k0 + VZ == n3) ? v[VZ - 1] : A(0);
+      const A s3[3] = {cs - last, cs, cs - first};
+#pragma unroll
+      for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+        for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+          for (int a3 = 0; a3 < 3; ++a3) acc[s * 27 + a1 * 9 + a2 * 3 + a3] += m1[a1] * m2[a2] * s3[a3];
+    }
+  }
+  __shared__ double sh[NV][4];
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 0; o < NV; ++o) {
+    const double s = wave_sum((double)acc[o]);
+    if ((threadIdx.x & 63) == 0) sh[o][w] = s;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < NV; o += kThreads)
+    partials[(size_t)blockIdx.x * NV + o] = sh[o][0] + sh[o][1] + sh[o][2] + sh[o][3];
+}
+
+template <typename A>
+__global__ __launch_bounds__(kThreads) void mean_patch_final_kernel(
+    const double* __restrict__ partials, int nblk, int NV, double invV, const A* __restrict__ w,
+    const A* __restrict__ b, A* __restrict__ z, A* __restrict__ ybar) {
+  __shared__ double sh[4];
+  double contrib = 0.0;
+  for (int o = threadIdx.x; o < NV; o += kThreads) {
+    double s = 0.0;
+    for (int blk = 0; blk < nblk; ++blk) s += partials[(size_t)blk * NV + o];
+    const A yb = (A)(s * invV);
+    ybar[o] = yb;
+    contrib += (double)w[o] * (double)yb;
+  }
+  const double tot = block_sum(contrib, sh);
+  if (threadIdx.x == 0) z[0] = (A)(tot + (double)b[0]);
+}
+
+static int mean_patch_blocks(int n1, int n2, int n3, int vz) {
+  const long nch = (long)n1 * n2 * (n3 / vz);
+  long nb = (nch + kThreads - 1) / kThreads;
+  if (nb > 1024) nb = 1024;
+  return (int)(nb < 1 ? 1 : nb);
+}
+
+static int mean_patch_vz(int n3) { return (n3 % 4 == 0) ? 4 : (n3 % 2 == 0 ? 2 : 1); }
+
+size_t mean_patch_workspace(int /*dtype*/, int C, int n1, int n2, int n3) {
+  return (size_t)mean_patch_blocks(n1, n2, n3, mean_patch_vz(n3)) * 27 * C * sizeof(double);
+}
+
+template <typename TI>
+static void launch_mp_t(const void* img, double* part, int nblk, int C, int n1, int n2, int n3, int vz,
+                        hipStream_t st) {
+#define MLPCG_MP(CC, VZZ)                                                                         \
+  if (C == CC && vz == VZZ) {                                                                     \
+    mean_patch_partial_kernel<TI, CC, VZZ><<<nblk, kThreads, 0, st>>>((const TI*)img, n1, n2, n3, \
+                                                                      part);                      \
+    return;                                                                                       \
+  }
+  MLPCG_MP(1, 4) MLPCG_MP(1, 2) MLPCG_MP(1, 1)
+  MLPCG_MP(2, 4) MLPCG_MP(2, 2) MLPCG_MP(2, 1)
+  MLPCG_MP(3, 4) MLPCG_MP(3, 2) MLPCG_MP(3, 1)
+#undef MLPCG_MP
+}
+
+int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,
+                     void* workspace, int C, int n1, int n2, int n3, hipStream_t st) {
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "sm_linear: C must be 1, 2 or 3 (got %d)", C);
+  const int vz = mean_patch_vz(n3);
+  const int nblk = mean_patch_blocks(n1, n2, n3, vz);
+  double* part = (double*)workspace;
+  const double invV = 1.0 / ((double)n1 * n2 * n3);
+  switch (dtype) {
+    case kF64: launch_mp_t<double>(img, part, nblk, C, n1, n2, n3, vz, st); break;
+    case kF32: launch_mp_t<float>(img, part, nblk, C, n1, n2, n3, vz, st); break;
+    case kF16: launch_mp_t<f16>(img, part, nblk, C, n1, n2, n3, vz, st); break;
+    case kBF16: launch_mp_t<bf16>(img, part, nblk, C, n1, n2, n3, vz, st); break;
+    default: MLPCG_REQUIRE(false, "sm_linear: bad dtype %d", dtype);
+  }
+  if (dtype == kF64)
+    mean_patch_final_kernel<double><<<1, kThreads, 0, st>>>(part, nblk, 27 * C, invV, (const double*)w,
+                                                            (const double*)b, (double*)z, (double*)ybar);
+  else
+    mean_patch_final_kernel<float><<<1, kThreads, 0, st>>>(part, nblk, 27 * C, invV, (const float*)w,
+                                                           (const float*)b, (float*)z, (float*)ybar);
+  return check_launch("sm_linear");
+}
+
+// gw = g * ybar, gb = g  (sm_linear_3d_kernel.cu:229-238)
+template <typename A>
+__global__ void linear_backward_kernel(const A* __restrict__ g, const A* __restrict__ ybar,
+                                       A* __restrict__ gw, A* __restrict__ gb, int NV) {
+  const A gv = g[0];
+  for (int o = threadIdx.x; o < NV; o += blockDim.x) gw[o] = gv * ybar[o];
+  if (threadIdx.x == 0) gb[0] = gv;
+}
+
+int launch_linear_backward(int dtype, const void* g, const void* ybar, void* gw, void* gb, int C,
+                           hipStream_t st) {
+  if (dtype == kF64)
+    linear_backward_kernel<double><<<1, 128, 0, st>>>((const double*)g, (const double*)ybar,
+                                                      (double*)gw, (double*)gb, 27 * C);
+  else
+    linear_backward_kernel<float><<<1, 128, 0, st>>>((const float*)g, (const float*)ybar, (float*)gw,
+                                                     (float*)gb, 27 * C);
+  return check_launch("sm_linear_backward");
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient.
+// ---------------------------------------------------------------------------
+template <int C> struct WGShape {
+  static constexpr int NQ = 27 * C + 1;          // patch columns + bias column
+  static constexpr int RP = 3, RQ = 4;           // register tile
+  static constexpr int NPG = 9;                  // 27 / RP
+  static constexpr int NQG = (NQ + RQ - 1) / RQ;
+  static constexpr int NQP = NQG * RQ;           // padded patch rows in LDS
+  static constexpr int NT = NPG * NQG;           // threads per voxel subset
+  static constexpr int NSUB = kThreads / NT;     // voxel subsets
+  static constexpr int NO = 27 * NQ;             // outputs
+};
+
+template <typename TX, typename TI, int C, int TV>
+__global__ __launch_bounds__(kThreads) void weight_grad_partial_kernel(
+    const TX* __restrict__ a, const TX* __restrict__ cv, int B, const TI* __restrict__ img, int n1,
+    int n2, int n3, typename AccOf<TX>::type* __restrict__ partials) {
+  typedef typename AccOf<TX>::type A;
+  typedef WGShape<C> S;
+  constexpr int LD = TV + 1;
+  __shared__ A Gs[27 * LD];
+  __shared__ A Ps[S::NQP * LD];
+  static_assert(S::NSUB >= 1, "tile too large");
+  static_assert(27 * LD >= kThreads * S::RP * S::RQ, "reduction buffer too small");
+  const int tid = threadIdx.x;
+  const int tile_id = tid % S::NT, sub = tid / S::NT;
+  const int pg = tile_id / S::NQG, qg = tile_id % S::NQG;
+  const long V = (long)n1 * n2 * n3;
+  const long s1 = (long)n2 * n3;
+  A acc[S::RP][S::RQ];
+#pragma unroll
+  for (int r = 0; r < S::RP; ++r)
+#pragma unroll
+    for (int q = 0; q < S::RQ; ++q) acc[r][q] = A(0);
+  for (int r = S::NQ; r < S::NQP; ++r)
+    for (int u = tid; u < TV; u += kThreads) Ps[r * LD + u] = A(0);
+
+  const long ntiles = (V + TV - 1) / TV;
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (tid < TV) {
+      const long v = tile * TV + tid;
+      A G[27];
+      A P[27 * C];
+#pragma unroll
+      for (int p = 0; p < 27; ++p) G[p] = A(0);
+#pragma unroll
+      for (int q = 0; q < 27 * C; ++q) P[q] = A(0);
+      if (v < V) {
+        const int k = (int)(v % n3);
+        const int j = (int)((v / n3) % n2);
+        const int i = (int)(v / s1);
+        bool in[27];
+        long off[27];
+#pragma unroll
+        for (int p = 0; p < 27; ++p) {
+          const int d1 = p / 9 - 1, d2 = (p / 3) % 3 - 1, d3 = p % 3 - 1;
+          in[p] = (unsigned)(i + d1) < (unsigned)n1 && (unsigned)(j + d2) < (unsigned)n2 &&
+                  (unsigned)(k + d3) < (unsigned)n3;
+          off[p] = d1 * s1 + (long)d2 * n3 + d3;
+        }
+        for (int b = 0; b < B; ++b) {
+          const A av = (A)a[(long)b * V + v];
+          const TX* cb = cv + (long)b * V + v;
+#pragma unroll
+          for (int p = 0; p < 27; ++p)
+            if (in[p]) G[p] += av * (A)cb[off[p]];
+        }
+#pragma unroll
+        for (int s = 0; s < C; ++s)
+#pragma unroll
+          for (int p = 0; p < 27; ++p)
+            if (in[p]) P[s * 27 + p] = (A)img[s * V + v + off[p]];
+      }
+#pragma unroll
+      for (int p = 0; p < 27; ++p) Gs[p * LD + tid] = G[p];
+#pragma unroll
+      for (int q = 0; q < 27 * C; ++q) Ps[q * LD + tid] = P[q];
+      Ps[(27 * C) * LD + tid] = (v < V) ? A(1) : A(0);
+    }
+    __syncthreads();
+    if (sub < S::NSUB) {
+      for (int u = sub; u < TV; u += S::NSUB) {
+        A g[S::RP], pv[S::RQ];
+#pragma unroll
+        for (int r = 0; r < S::RP; ++r) g[r] = Gs[(pg * S::RP + r) * LD + u];
+#pragma unroll
+        for (int q = 0; q < S::RQ; ++q) pv[q] = Ps[(qg * S::RQ + q) * LD + u];
+#pragma unroll
+        for (int r = 0; r < S::RP; ++r)
+#pragma unroll
+          for (int q = 0; q < S::RQ; ++q) acc[r][q] += g[r] * pv[q];
+      }
+    }
+    __syncthreads();
+  }
+  A* red = Gs;
+  if (sub < S::NSUB) {
+#pragma unroll
+    for (int r = 0; r < S::RP; ++r)
+#pragma unroll
+      for (int q = 0; q < S::RQ; ++q)
+        red[((sub * S::NT + tile_id) * S::RP + r) * S::RQ + q] = acc[r][q];
+  }
+  __syncthreads();
+  if (tid < S::NT) {
+#pragma unroll
+    for (int r = 0; r < S::RP; ++r) {
+#pragma unroll
+      for (int q = 0; q < S::RQ; ++q) {
+        A s = A(0);
+        for (int sb = 0; sb < S::NSUB; ++sb) s += red[((sb * S::NT + tile_id) * S::RP + r) * S::RQ + q];
+        const int op = pg * S::RP + r, oq = qg * S::RQ + q;
+        if (oq < S::NQ) partials[(size_t)blockIdx.x * S::NO + op * S::NQ + oq] = s;
+      }
+    }
+  }
+}
+
+template <typename A>
+__global__ __launch_bounds__(kThreads) void weight_grad_final_kernel(const A* __restrict__ partials,
+                                                                     int nblk, int C,
+                                                                     A* __restrict__ gW,
+                                                                     A* __restrict__ gb) {
+  const int NQ = 27 * C + 1, NO = 27 * NQ;
+  const int o = blockIdx.x * kThreads + threadIdx.x;
+  if (o >= NO) return;
+  double s = 0.0;
+  for (int blk = 0; blk < nblk; ++blk) s += (double)partials[(size_t)blk * NO + o];
+  const int p = o / NQ, q = o % NQ;
+  if (q < 27 * C) gW[p * 27 * C + q] = (A)s;
+  else gb[p] = (A)s;
+}
+
+static int wg_tv(int dtype, int C) { return (dtype == kF64 && C >= 2) ? 128 : 256; }
+
+static int wg_blocks(int dtype, int C, int n1, int n2, int n3) {
+  const long V = (long)n1 * n2 * n3;
+  const int tv = wg_tv(dtype, C);
+  long nb = (V + tv - 1) / tv;
+  if (nb > 1024) nb = 1024;
+  return (int)(nb < 1 ? 1 : nb);
+}
+
+size_t weight_grad_workspace(int dtype, int C, int /*B*/, int n1, int n2, int n3) {
+  const size_t acc = dtype == kF64 ? sizeof(double) : sizeof(float);
+  return (size_t)wg_blocks(dtype, C, n1, n2, n3) * 27 * (27 * C + 1) * acc;
+}
+
+template <typename TX, typename TI>
+static void launch_wg_t(int dtype, const void* a, const void* c, const void* img, void* part, int nblk,
+                        int C, int B, int n1, int n2, int n3, hipStream_t st) {
+  typedef typename AccOf<TX>::type A;
+  const int tv = wg_tv(dtype, C);
+#define MLPCG_WG(CC, TVV)                                                                  \
+  if (C == CC && tv == TVV) {                                                              \
+    weight_grad_partial_kernel<TX, TI, CC, TVV><<<nblk, kThreads, 0, st>>>(                \
+        (const TX*)a, (const TX*)c, B, (const TI*)img, n1, n2, n3, (A*)part);              \
+    return;                                                                                \
+  }
+  if constexpr (sizeof(A) == 8) {
+    MLPCG_WG(1, 256) MLPCG_WG(2, 128) MLPCG_WG(3, 128)
+  } else {
+    MLPCG_WG(1, 256) MLPCG_WG(2, 256) MLPCG_WG(3, 256)
+  }
+#undef MLPCG_WG
+}
+
+int launch_weight_grad(int dtype, const void* a, const void* c, const void* img, void* gW, void* gb,
+                       void* workspace, int C, int B, int n1, int n2, int n3, hipStream_t st) {
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "weight_grad: C must be 1, 2 or 3 (got %d)", C);
+  const int nblk = wg_blocks(dtype, C, n1, n2, n3);
+  switch (dtype) {
+    case kF64: launch_wg_t<double, double>(dtype, a, c, img, workspace, nblk, C, B, n1, n2, n3, st); break;
+    case kF32: launch_wg_t<float, float>(dtype, a, c, img, workspace, nblk, C, B, n1, n2, n3, st); break;
+    case kF16: launch_wg_t<f16, f16>(dtype, a, c, img, workspace, nblk, C, B, n1, n2, n3, st); break;
+    case kBF16: launch_wg_t<bf16, bf16>(dtype, a, c, img, workspace, nblk, C, B, n1, n2, n3, st); break;
+    default: MLPCG_REQUIRE(false, "weight_grad: bad dtype %d", dtype);
+  }
+  const int NO = 27 * (27 * C + 1);
+  const int fb = (NO + kThreads - 1) / kThreads;
+  if (dtype == kF64)
+    weight_grad_final_kernel<double><<<fb, kThreads, 0, st>>>((const double*)workspace, nblk, C,
+                                                              (double*)gW, (double*)gb);
+  else
+    weight_grad_final_kernel<float><<<fb, kThreads, 0, st>>>((const float*)workspace, nblk, C,
+                                                             (float*)gW, (float*)gb);
+  return check_launch("weight_grad");
+}
+
+}  // namespace mlpcg

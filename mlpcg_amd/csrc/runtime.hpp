@@ -1,0 +1,90 @@
+// Native runtime objects: the preconditioner plan and the PCG solver.
+#pragma once
+
+#include <vector>
+
+#include "kernels.hpp"
+#include "pcg.hpp"
+
+namespace mlpcg {
+
+constexpr int kMaxLevels = 10;
+
+enum ModelKind : int { kSmallSM = 0, kSPD = 1, kSmallSPD = 2 };
+
+// Everything that depends only on the image is computed once at creation: the image
+// pyramid (accumulate precision), the stencil weights K of every block (storage
+// precision, [27][V_level]) and the SMLinear coefficients c0/c1 (device scalars).
+// This is the caching that SmallSMModelDn3D.eval_forward does for imgs/c0/c1 through
+// its list arguments (sm_model_3d.py:180-195), extended to K.
+struct Precond {
+  int kind = kSmallSM;
+  int n = 0, C = 1, dtype = kF32, acc = kF32;
+  int nearest = 0, mask = 0, swap = 0;
+  int dims[kMaxLevels + 1][3];
+  long V[kMaxLevels + 1];
+  void* img[kMaxLevels + 1] = {};   // [C][V_l] accumulate type
+  std::vector<void*> K;             // stencils in block order (see precond.cpp)
+  void* coef = nullptr;             // accumulate type: c0[0..n-1], c1[0..n-1] (+ squares)
+  void* xd[kMaxLevels + 1] = {};    // downward level vectors (storage type)
+  void* xo[kMaxLevels + 1] = {};    // upward level vectors (storage type)
+  void* tmp[kMaxLevels + 1] = {};   // transfer scratch (storage type)
+  void* xs[kMaxLevels + 1] = {};    // SPD second-half vectors (storage type)
+  uint8_t* maskimg = nullptr;       // level-0 model mask: img[1] != 0
+  size_t bytes = 0;
+  std::vector<void*> allocs;
+
+  int apply(const float* r, float* z, const double* in_scale, const uint8_t* out_mask,
+            const int* done, hipStream_t st);
+  ~Precond();
+};
+
+int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3, int dtype,
+                   int nearest, int mask, int swap, const void* image, const void* const* weights,
+                   const void* const* biases, hipStream_t st);
+
+struct SolveInfo {
+  int iters;
+  int status;         // 0 converged, 1 max_it reached, 2 breakdown (non-finite)
+  double rel_res;     // ||r|| / ||b|| of the last FP64 residual recompute
+  double norm_b;
+};
+
+struct Solver {
+  int layout = 0;     // 0 flat grid (matrix-free), 1 CSR over the unknowns
+  int n1 = 0, n2 = 0, n3 = 0;
+  long V = 0, n = 0;
+  uint8_t* code = nullptr;
+  const int* rowptr = nullptr;
+  const int* col = nullptr;
+  const float* val = nullptr;
+  const int* fluid_idx = nullptr;
+  float* x[2] = {};
+  float* r[2] = {};
+  float* p[2] = {};
+  float* z = nullptr;
+  float* w = nullptr;
+  float* q_hist[2] = {};   // npsd: previous directions
+  float* aq_hist[2] = {};  // npsd: their A-products
+  float* grid_in = nullptr;
+  float* grid_out = nullptr;
+  SolveState* st = nullptr;
+  SolveState* st_host = nullptr;   // pinned ring of status snapshots
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  double* history = nullptr;
+  int hist_cap = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_iter[4] = {}, ev_stat[4] = {};
+  std::vector<void*> allocs;
+
+  int solve(Precond* P, const float* b, const float* x0, float* x_out, int method, int max_it,
+            double tol, double atol, double* history_out, SolveInfo* info, hipStream_t st);
+  ~Solver();
+};
+
+int solver_create_flat(Solver** out, const int8_t* flags, int n1, int n2, int n3, hipStream_t st);
+int solver_create_csr(Solver** out, const int* rowptr, const int* col, const float* val, long n,
+                      const int* fluid_idx, int n1, int n2, int n3, hipStream_t st);
+
+}  // namespace mlpcg

@@ -1,0 +1,294 @@
+// Native PCG driver (npcg / npcg_flex / npsd of cg_tests.py) over either the full grid
+// with the matrix-free Laplacian or CSR over the compressed unknowns.
+//
+// One iteration is a fixed sequence of kernels on the caller's stream; the scalars
+// (rho, beta, alpha, ||r||, the stop decision) live in a device SolveState updated by the
+// reductions' last blocks.  The host runs up to kLookahead iterations ahead of the GPU:
+// after queueing iteration k it waits only for the status of iteration k - kLookahead,
+// copied on a side stream.  Once the device sets `done`, the kernels of already-queued
+// iterations return immediately, so the iteration count and the result are exactly
+// those of a loop that checks after every iteration (cg_tests.py:70, :150, :274).
+#include <cmath>
+#include <cstring>
+
+#include "runtime.hpp"
+
+namespace mlpcg {
+
+constexpr int kLookahead = 2;
+
+Solver::~Solver() {
+  if (side) (void)hipStreamSynchronize(side);
+  for (int i = 0; i < 4; ++i) {
+    if (ev_iter[i]) (void)hipEventDestroy(ev_iter[i]);
+    if (ev_stat[i]) (void)hipEventDestroy(ev_stat[i]);
+  }
+  if (side) (void)hipStreamDestroy(side);
+  if (st_host) (void)hipHostFree(st_host);
+  for (void* p : allocs) (void)hipFree(p);
+}
+
+static int salloc(Solver* S, void** out, size_t bytes, bool zero, hipStream_t st) {
+  if (bytes == 0) bytes = 16;
+  MLPCG_HIP(hipMalloc(out, bytes));
+  S->allocs.push_back(*out);
+  if (zero) MLPCG_HIP(hipMemsetAsync(*out, 0, bytes, st));
+  return 0;
+}
+
+static int solver_common(Solver* S, hipStream_t st) {
+  const size_t vb = (size_t)S->n * sizeof(float);
+  for (int i = 0; i < 2; ++i) {
+    if (salloc(S, (void**)&S->x[i], vb, true, st)) return -1;
+    if (salloc(S, (void**)&S->r[i], vb, true, st)) return -1;
+    if (salloc(S, (void**)&S->p[i], vb, true, st)) return -1;
+    if (salloc(S, (void**)&S->q_hist[i], vb, true, st)) return -1;
+    if (salloc(S, (void**)&S->aq_hist[i], vb, true, st)) return -1;
+  }
+  if (salloc(S, (void**)&S->z, vb, true, st)) return -1;
+  if (salloc(S, (void**)&S->w, vb, true, st)) return -1;
+  if (salloc(S, (void**)&S->st, sizeof(SolveState), true, st)) return -1;
+  if (salloc(S, (void**)&S->partials, reduction_workspace_bytes(), true, st)) return -1;
+  if (salloc(S, (void**)&S->counter, 256, true, st)) return -1;
+  MLPCG_HIP(hipHostMalloc((void**)&S->st_host, 4 * sizeof(SolveState), hipHostMallocDefault));
+  memset(S->st_host, 0, 4 * sizeof(SolveState));
+  MLPCG_HIP(hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking));
+  for (int i = 0; i < 4; ++i) {
+    MLPCG_HIP(hipEventCreateWithFlags(&S->ev_iter[i], hipEventDisableTiming));
+    MLPCG_HIP(hipEventCreateWithFlags(&S->ev_stat[i], hipEventDisableTiming));
+  }
+  return 0;
+}
+
+int solver_create_flat(Solver** out, const int8_t* flags, int n1, int n2, int n3, hipStream_t st) {
+  MLPCG_REQUIRE(out && flags, "solver_create_flat: null argument");
+  MLPCG_REQUIRE(n1 > 0 && n2 > 0 && n3 > 0, "solver_create_flat: bad grid");
+  Solver* S = new Solver();
+  S->layout = 0;
+  S->n1 = n1;
+  S->n2 = n2;
+  S->n3 = n3;
+  S->V = S->n = (long)n1 * n2 * n3;
+  if (salloc(S, (void**)&S->code, (size_t)S->V, false, st) || solver_common(S, st) ||
+      launch_flags_code(flags, S->code, n1, n2, n3, st)) {
+    delete S;
+    return -1;
+  }
+  *out = S;
+  return 0;
+}
+
+int solver_create_csr(Solver** out, const int* rowptr, const int* col, const float* val, long n,
+                      const int* fluid_idx, int n1, int n2, int n3, hipStream_t st) {
+  MLPCG_REQUIRE(out && rowptr && col && val && fluid_idx, "solver_create_csr: null argument");
+  MLPCG_REQUIRE(n > 0, "solver_create_csr: empty system");
+  Solver* S = new Solver();
+  S->layout = 1;
+  S->n1 = n1;
+  S->n2 = n2;
+  S->n3 = n3;
+  S->V = (long)n1 * n2 * n3;
+  S->n = n;
+  S->rowptr = rowptr;
+  S->col = col;
+  S->val = val;
+  S->fluid_idx = fluid_idx;
+  if (solver_common(S, st) || salloc(S, (void**)&S->grid_in, (size_t)S->V * sizeof(float), true, st) ||
+      salloc(S, (void**)&S->grid_out, (size_t)S->V * sizeof(float), true, st)) {
+    delete S;
+    return -1;
+  }
+  *out = S;
+  return 0;
+}
+
+int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int method, int max_it,
+                  double tol, double atol, double* history_out, SolveInfo* info, hipStream_t st) {
+  MLPCG_REQUIRE(b && x_out, "solve: null vector");
+  MLPCG_REQUIRE(method >= kNpcg && method <= kNpsd, "solve: bad method %d", method);
+  MLPCG_REQUIRE(max_it >= 1, "solve: max_it must be >= 1");
+  if (P) {
+    MLPCG_REQUIRE(P->dims[0][0] == n1 && P->dims[0][1] == n2 && P->dims[0][2] == n3,
+                  "solve: preconditioner grid does not match the solver grid");
+    MLPCG_REQUIRE(P->dtype != kF64, "solve: FP64 plans are for checking only; use f32/bf16/f16");
+  }
+  if (history_out && hist_cap < max_it + 1) {
+    void* h = nullptr;
+    if (salloc(this, &h, (size_t)(max_it + 1) * sizeof(double), false, st)) return -1;
+    history = (double*)h;
+    hist_cap = max_it + 1;
+  }
+  // initial state
+  SolveState init;
+  memset(&init, 0, sizeof(init));
+  init.tol = tol;
+  init.atol = atol;
+  init.max_it = max_it;
+  init.method = method;
+  init.history = history_out ? history : nullptr;
+  SolveState* hs = st_host;  // slot 0 carries the initial state up
+  MLPCG_HIP(hipStreamSynchronize(side));
+  *hs = init;
+  MLPCG_HIP(hipMemcpyAsync(this->st, hs, sizeof(SolveState), hipMemcpyHostToDevice, st));
+  const size_t vb = (size_t)n * sizeof(float);
+  if (x0) MLPCG_HIP(hipMemcpyAsync(x[0], x0, vb, hipMemcpyDeviceToDevice, st));
+  else MLPCG_HIP(hipMemsetAsync(x[0], 0, vb, st));
+  MLPCG_HIP(hipMemsetAsync(p[0], 0, vb, st));
+  MLPCG_HIP(hipMemsetAsync(p[1], 0, vb, st));
+  const int* done = &this->st->done;
+
+  FlatArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  fa.code = code;
+  fa.b = b;
+  fa.st = this->st;
+  fa.partials = partials;
+  fa.counter = counter;
+  fa.n1 = n1;
+  fa.n2 = n2;
+  fa.n3 = n3;
+  CsrArgs ca;
+  memset(&ca, 0, sizeof(ca));
+  ca.rowptr = rowptr;
+  ca.col = col;
+  ca.val = val;
+  ca.b = b;
+  ca.st = this->st;
+  ca.partials = partials;
+  ca.counter = counter;
+  ca.n = n;
+
+  int rc = 0;
+#define TRY(x)         \
+  do {                 \
+    rc = (x);          \
+    if (rc) return rc; \
+  } while (0)
+  // r_0 = b - A x_0
+  if (layout == 0) {
+    fa.x_old = x[0];
+    fa.r = r[0];
+    TRY(launch_resid_flat(fa, 1, st));
+  } else {
+    ca.x_old = x[0];
+    ca.r = r[0];
+    TRY(launch_resid_csr(ca, 1, st));
+  }
+
+  auto precondition = [&](const float* rin, float* zout) -> int {
+    if (!P) {
+      MLPCG_HIP(hipMemcpyAsync(zout, rin, vb, hipMemcpyDeviceToDevice, st));
+      return 0;
+    }
+    const double* scale = &this->st->scale;
+    if (layout == 0) return P->apply(rin, zout, scale, code, done, st);
+    int rc2 = launch_scatter(fluid_idx, rin, grid_in, n, done, st);
+    if (rc2) return rc2;
+    rc2 = P->apply(grid_in, grid_out, scale, nullptr, done, st);
+    if (rc2) return rc2;
+    return launch_gather(fluid_idx, grid_out, zout, n, done, st);
+  };
+
+  int k = 0;
+  int npsd_count = 0;  // directions kept (npsd)
+  for (k = 1; k <= max_it; ++k) {
+    const int cur = (k - 1) & 1, nxt = k & 1;
+    const float* rk = r[cur];
+    if (method != kNpsd) {
+      TRY(precondition(rk, z));
+      TRY(launch_rz_dot(rk, z, r[nxt], n, this->st, partials, counter, st));
+      if (layout == 0) {
+        fa.z = z;
+        fa.p_old = p[cur];
+        fa.p_new = p[nxt];
+        fa.w = w;
+        TRY(launch_p_spmv_flat(fa, st));
+        fa.x_old = x[cur];
+        fa.x_new = x[nxt];
+        fa.p = p[nxt];
+        fa.r = r[nxt];
+        TRY(launch_resid_flat(fa, 0, st));
+      } else {
+        ca.z = z;
+        ca.p_old = p[cur];
+        ca.p_new = p[nxt];
+        ca.w = w;
+        TRY(launch_p_spmv_csr(ca, st));
+        ca.x_old = x[cur];
+        ca.x_new = x[nxt];
+        ca.p = p[nxt];
+        ca.r = r[nxt];
+        TRY(launch_resid_csr(ca, 0, st));
+      }
+    } else {
+      // q = P(r); q -= (q.Ap_j / alfa_j) p_j for the kept directions, newest first
+      float* q = z;
+      TRY(precondition(rk, q));
+      const int kept = npsd_count < 2 ? npsd_count : 2;
+      for (int jj = kept - 1; jj >= 0; --jj) {
+        const int slot = (npsd_count - kept + jj) & 1;  // ring position of direction jj
+        TRY(launch_npsd_proj(q, aq_hist[slot], &this->st->alfa[slot], n, this->st, partials, counter, st));
+        TRY(launch_npsd_axpy(q, q_hist[slot], n, this->st, st));
+      }
+      // the new direction replaces the oldest slot
+      const int slot_new = npsd_count & 1;
+      MLPCG_HIP(hipMemcpyAsync(q_hist[slot_new], q, vb, hipMemcpyDeviceToDevice, st));
+      if (layout == 0) {
+        fa.z = q_hist[slot_new];
+        fa.r = const_cast<float*>(rk);
+        fa.w = aq_hist[slot_new];
+        TRY(launch_npsd_spmv_flat(fa, &this->st->alfa[slot_new], st));
+        fa.x_old = x[cur];
+        fa.x_new = x[nxt];
+        fa.p = q_hist[slot_new];
+        fa.r = r[nxt];
+        TRY(launch_resid_flat(fa, 0, st));
+      } else {
+        ca.z = q_hist[slot_new];
+        ca.r = const_cast<float*>(rk);
+        ca.w = aq_hist[slot_new];
+        TRY(launch_npsd_spmv_csr(ca, &this->st->alfa[slot_new], st));
+        ca.x_old = x[cur];
+        ca.x_new = x[nxt];
+        ca.p = q_hist[slot_new];
+        ca.r = r[nxt];
+        TRY(launch_resid_csr(ca, 0, st));
+      }
+      ++npsd_count;
+    }
+    // status of iteration k goes to the host on the side stream
+    const int e = k & 3;
+    MLPCG_HIP(hipEventRecord(ev_iter[e], st));
+    MLPCG_HIP(hipStreamWaitEvent(side, ev_iter[e], 0));
+    MLPCG_HIP(hipMemcpyAsync(&st_host[e], this->st, sizeof(SolveState), hipMemcpyDeviceToHost, side));
+    MLPCG_HIP(hipEventRecord(ev_stat[e], side));
+    if (k > kLookahead) {
+      const int e2 = (k - kLookahead) & 3;
+      MLPCG_HIP(hipEventSynchronize(ev_stat[e2]));
+      if (st_host[e2].done) break;
+    }
+  }
+#undef TRY
+  // final state
+  MLPCG_HIP(hipStreamSynchronize(st));
+  MLPCG_HIP(hipStreamSynchronize(side));
+  SolveState fin;
+  MLPCG_HIP(hipMemcpy(&fin, this->st, sizeof(SolveState), hipMemcpyDeviceToHost));
+  const int iters = fin.iter;
+  MLPCG_HIP(hipMemcpyAsync(x_out, x[iters & 1], vb, hipMemcpyDeviceToDevice, st));
+  if (history_out)
+    MLPCG_HIP(hipMemcpy(history_out, history, (size_t)(iters + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  MLPCG_HIP(hipStreamSynchronize(st));
+  if (info) {
+    info->iters = iters;
+    info->rel_res = fin.norm;
+    info->norm_b = fin.norm_b;
+    const double tolv = fmax(tol, atol / fin.norm_b);
+    if (!std::isfinite(fin.norm)) info->status = 2;
+    else if ((method == kNpsd) ? (fin.norm < tolv) : (fin.norm <= tolv)) info->status = 0;
+    else info->status = 1;
+  }
+  return 0;
+}
+
+}  // namespace mlpcg

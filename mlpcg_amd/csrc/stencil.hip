@@ -1,0 +1,573 @@
+// Spatially varying 3x3x3 stencil (SM block) kernels for gfx950.
+//
+//   stencil_weights  K_p(v) = b_p + sum_{s,e} W[p,s,e] img[s,v+e]                 (27 x V)
+//                    (the per-voxel "N x 27 x (27C+1)" contraction; the reference
+//                    recomputes it inside every block kernel, sm_block_3d_kernel.cu:46-69)
+//                    transposed form Kt_q(v) = K_{26-q}(v + d_q), so that the adjoint
+//                    block (sm_block_trans_3d_kernel.cu:46-73) is the same apply kernel.
+//   sm_apply         y[b,v] = sum_p K_p(v) x[b, v+d_p]  with a fused epilogue
+//                    (input scale, model mask, y = c0*prev + c1*y, output mask).
+//   avgpool2 / upsample2 / upsample2_adjoint   level transfers
+//                    (F.avg_pool3d(.,2) and F.interpolate(scale 2, align_corners=False)).
+//
+// Thread mapping everywhere: one thread owns VZ consecutive voxels along the fastest
+// axis; a wavefront therefore streams 64*VZ consecutive voxels of every K tap.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace mlpcg {
+
+// Row segment x[ii][jj][k0-1 .. k0+VZ] (VZ+2 values), zero outside the grid.
+template <int VZ, typename A, typename T>
+__device__ __forceinline__ void load_row(const T* __restrict__ base, int ii, int jj, int k0,
+                                         int n1, int n2, int n3, A* row) {
+  if ((unsigned)ii < (unsigned)n1 && (unsigned)jj < (unsigned)n2) {
+    const T* r = base + ((long)ii * n2 + jj) * n3;
+    load_vec<VZ>(r + k0, row + 1);
+    row[0] = k0 > 0 ? (A)r[k0 - 1] : A(0);
+    row[VZ + 1] = (k0 + VZ < n3) ? (A)r[k0 + VZ] : A(0);
+  } else {
+#pragma unroll
+    for (int t = 0; t < VZ + 2; ++t) row[t] = A(0);
+  }
+}
+
+// Row segment x[ii][jj][k0-2 .. k0+VZ+1] (VZ+4 values), zero outside the grid.
+template <int VZ, typename A, typename T>
+__device__ __forceinline__ void load_row2(const T* __restrict__ base, int ii, int jj, int k0,
+                                          int n1, int n2, int n3, A* row) {
+  if ((unsigned)ii < (unsigned)n1 && (unsigned)jj < (unsigned)n2) {
+    const T* r = base + ((long)ii * n2 + jj) * n3;
+    load_vec<VZ>(r + k0, row + 2);
+    row[0] = k0 > 1 ? (A)r[k0 - 2] : A(0);
+    row[1] = k0 > 0 ? (A)r[k0 - 1] : A(0);
+    row[VZ + 2] = (k0 + VZ < n3) ? (A)r[k0 + VZ] : A(0);
+    row[VZ + 3] = (k0 + VZ + 1 < n3) ? (A)r[k0 + VZ + 1] : A(0);
+  } else {
+#pragma unroll
+    for (int t = 0; t < VZ + 4; ++t) row[t] = A(0);
+  }
+}
+
+struct ChunkPos {
+  int i, j, k0;
+  long v0;
+};
+
+template <int VZ>
+__device__ __forceinline__ bool chunk_pos(long cid, int n1, int n2, int n3, ChunkPos& c) {
+  const int n3c = n3 / VZ;
+  const long nch = (long)n1 * n2 * n3c;
+  if (cid >= nch) return false;
+  const int kc = (int)(cid % n3c);
+  const long t = cid / n3c;
+  c.j = (int)(t % n2);
+  c.i = (int)(t / n2);
+  c.k0 = kc * VZ;
+  c.v0 = ((long)c.i * n2 + c.j) * n3 + c.k0;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// stencil_weights
+// ---------------------------------------------------------------------------
+template <typename TI, typename TK, int C, int VZ>
+__global__ __launch_bounds__(kThreads) void stencil_weights_kernel(
+    const TI* __restrict__ img, const typename AccOf<TK>::type* __restrict__ W,
+    const typename AccOf<TK>::type* __restrict__ bias, TK* __restrict__ K, int n1, int n2, int n3) {
+  typedef typename AccOf<TK>::type A;
+  ChunkPos c;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (!chunk_pos<VZ>(cid, n1, n2, n3, c)) return;
+  const long V = (long)n1 * n2 * n3;
+  // the 3x3 neighbourhood rows of every channel stay in registers; the taps are a
+  // runtime loop so each tap's 27C weights stream through the scalar cache
+  A rows[C][3][3][VZ + 2];
+#pragma unroll
+  for (int s = 0; s < C; ++s)
+#pragma unroll
+    for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+      for (int a2 = 0; a2 < 3; ++a2)
+        load_row<VZ>(img + s * V, c.i + a1 - 1, c.j + a2 - 1, c.k0, n1, n2, n3, rows[s][a1][a2]);
+#pragma unroll 1
+  for (int p = 0; p < 27; ++p) {
+    const A* __restrict__ Wp = W + p * C * 27;
+    A acc[VZ];
+    const A bp = bias[p];
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) acc[t] = bp;
+#pragma unroll
+    for (int s = 0; s < C; ++s)
+#pragma unroll
+      for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+        for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+          for (int a3 = 0; a3 < 3; ++a3) {
+            const A w = Wp[s * 27 + a1 * 9 + a2 * 3 + a3];
+#pragma unroll
+            for (int t = 0; t < VZ; ++t) acc[t] += w * rows[s][a1][a2][t + a3];
+          }
+    store_vec<VZ>(K + p * V + c.v0, acc);
+  }
+}
+
+// Kt_q(v) = K_p(v - d_p) with p = 26 - q (so d_q = -d_p); zero where v + d_q is outside.
+template <typename TI, typename TK, int C, int VZ>
+__global__ __launch_bounds__(kThreads) void stencil_weights_t_kernel(
+    const TI* __restrict__ img, const typename AccOf<TK>::type* __restrict__ W,
+    const typename AccOf<TK>::type* __restrict__ bias, TK* __restrict__ K, int n1, int n2, int n3) {
+  typedef typename AccOf<TK>::type A;
+  ChunkPos c;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (!chunk_pos<VZ>(cid, n1, n2, n3, c)) return;
+  const long V = (long)n1 * n2 * n3;
+  // runtime loop over the 9 (dq1, dq2) columns of output taps; for each, the image
+  // rows around (i+dq1, j+dq2) are read with a two-voxel z halo that serves all three
+  // dq3 taps of the column
+#pragma unroll 1
+  for (int g = 0; g < 9; ++g) {
+    const int dq1 = g / 3 - 1, dq2 = g % 3 - 1;
+    A acc[3][VZ];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const A bp = bias[26 - (g * 3 + u)];
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) acc[u][t] = bp;
+    }
+#pragma unroll
+    for (int s = 0; s < C; ++s) {
+#pragma unroll
+      for (int e1 = 0; e1 < 3; ++e1) {
+#pragma unroll
+        for (int e2 = 0; e2 < 3; ++e2) {
+          A row[VZ + 4];
+          load_row2<VZ>(img + s * V, c.i + dq1 + e1 - 1, c.j + dq2 + e2 - 1, c.k0, n1, n2, n3, row);
+#pragma unroll
+          for (int u = 0; u < 3; ++u) {  // dq3 = u - 1, q = 3g + u, p = 26 - q
+            const A* __restrict__ Wp = W + ((26 - (g * 3 + u)) * C + s) * 27 + e1 * 9 + e2 * 3;
+#pragma unroll
+            for (int e3 = 0; e3 < 3; ++e3) {
+              const A w = Wp[e3];
+#pragma unroll
+              for (int t = 0; t < VZ; ++t) acc[u][t] += w * row[t + 1 + u + e3 - 1];
+            }
+          }
+        }
+      }
+    }
+    const bool rowok = (unsigned)(c.i + dq1) < (unsigned)n1 && (unsigned)(c.j + dq2) < (unsigned)n2;
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) {
+        const bool ok = rowok && (unsigned)(c.k0 + t + u - 1) < (unsigned)n3;
+        if (!ok) acc[u][t] = A(0);
+      }
+      store_vec<VZ>(K + (long)(g * 3 + u) * V + c.v0, acc[u]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sm_apply
+// ---------------------------------------------------------------------------
+template <typename TK, typename TX, typename TY, typename TP, int VZ, int BB>
+__global__ __launch_bounds__(kThreads) void sm_apply_kernel(ApplyArgs a) {
+  typedef typename AccOf<TK>::type A;
+  if (solve_done(a.done)) return;
+  ChunkPos c;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (!chunk_pos<VZ>(cid, a.n1, a.n2, a.n3, c)) return;
+  const long V = (long)a.n1 * a.n2 * a.n3;
+  const int b0 = blockIdx.y * BB;
+  const TK* __restrict__ K = (const TK*)a.K;
+  const TX* __restrict__ x = (const TX*)a.x;
+  A acc[BB][VZ];
+#pragma unroll
+  for (int bb = 0; bb < BB; ++bb)
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) acc[bb][t] = A(0);
+
+#pragma unroll
+  for (int a1 = 0; a1 < 3; ++a1) {
+#pragma unroll
+    for (int a2 = 0; a2 < 3; ++a2) {
+      A kv[3][VZ];
+#pragma unroll
+      for (int a3 = 0; a3 < 3; ++a3)
+        load_vec_nt<VZ>(K + (long)(9 * a1 + 3 * a2 + a3) * V + c.v0, kv[a3]);
+#pragma unroll
+      for (int bb = 0; bb < BB; ++bb) {
+        if (BB > 1 && b0 + bb >= a.B) break;
+        A row[VZ + 2];
+        load_row<VZ>(x + (long)(b0 + bb) * V, c.i + a1 - 1, c.j + a2 - 1, c.k0, a.n1, a.n2,
+                     a.n3, row);
+#pragma unroll
+        for (int t = 0; t < VZ; ++t)
+          acc[bb][t] += kv[0][t] * row[t] + kv[1][t] * row[t + 1] + kv[2][t] * row[t + 2];
+      }
+    }
+  }
+
+  // epilogue
+  const A scale = a.in_scale ? (A)(*a.in_scale) : A(1);
+  A keep_a[VZ], keep_b[VZ];
+  if (a.mask_a) load_vec<VZ>(a.mask_a + c.v0, keep_a);
+  if (a.mask_b) load_vec<VZ>(a.mask_b + c.v0, keep_b);
+  const A c0 = a.prev ? *(const A*)a.c0 : A(0);
+  const A c1 = a.prev ? *(const A*)a.c1 : A(0);
+#pragma unroll
+  for (int bb = 0; bb < BB; ++bb) {
+    if (BB > 1 && b0 + bb >= a.B) break;
+    const long off = (long)(b0 + bb) * V + c.v0;
+    A out[VZ];
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) {
+      A val = acc[bb][t] * scale;
+      if (a.mask_a && keep_a[t] == A(0)) val = A(0);
+      out[t] = val;
+    }
+    if (a.prev) {
+      A pv[VZ];
+      load_vec<VZ>((const TP*)a.prev + off, pv);
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) out[t] = c0 * pv[t] + c1 * out[t];
+    }
+    if (a.mask_b) {
+#pragma unroll
+      for (int t = 0; t < VZ; ++t)
+        if (keep_b[t] == A(0)) out[t] = A(0);
+    }
+    store_vec<VZ>((TY*)a.y + off, out);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Level transfers
+// ---------------------------------------------------------------------------
+// out[b][I][J][K] = mean of the 2x2x2 fine cells; (n1, n2, n3) are the coarse dims.
+template <typename TI, typename TO, int VZ>
+__global__ __launch_bounds__(kThreads) void avgpool2_kernel(const TI* __restrict__ in,
+                                                            TO* __restrict__ out, int B, int n1,
+                                                            int n2, int n3, const int* done) {
+  typedef typename AccOf<TO>::type A;
+  if (solve_done(done)) return;
+  ChunkPos c;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (!chunk_pos<VZ>(cid, n1, n2, n3, c)) return;
+  const int b = blockIdx.y;
+  const long Vc = (long)n1 * n2 * n3;
+  const int f2 = 2 * n2, f3 = 2 * n3;
+  const long Vf = Vc * 8;
+  A s[VZ];
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) s[t] = A(0);
+#pragma unroll
+  for (int a1 = 0; a1 < 2; ++a1) {
+#pragma unroll
+    for (int a2 = 0; a2 < 2; ++a2) {
+      A row[2 * VZ];
+      load_vec<2 * VZ>(in + b * Vf + ((long)(2 * c.i + a1) * f2 + 2 * c.j + a2) * f3 + 2 * c.k0, row);
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) s[t] += row[2 * t] + row[2 * t + 1];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) s[t] *= A(0.125);
+  store_vec<VZ>(out + b * Vc + c.v0, s);
+}
+
+// Linear 1-D source indices for the x2 upsample with align_corners=False:
+// src = max((f + 0.5) / 2 - 0.5, 0); i0 = floor(src), i1 = min(i0 + 1, nc - 1).
+__device__ __forceinline__ void up_axis(int f, int nc, bool nearest, int& i0, int& i1, float& lam) {
+  if (nearest) {
+    i0 = i1 = f >> 1;
+    lam = 0.f;
+    return;
+  }
+  const int m = f >> 1;
+  if (f & 1) {
+    i0 = m;
+    lam = 0.25f;
+  } else if (m == 0) {
+    i0 = 0;
+    lam = 0.f;
+  } else {
+    i0 = m - 1;
+    lam = 0.75f;
+  }
+  i1 = min(i0 + 1, nc - 1);
+}
+
+// out = scale * up(in)      or, with prev:  out = c0 * prev + c1 * scale * up(in)
+// (n1, n2, n3) are the FINE dims.
+template <typename TI, typename TO, typename TP, int VZ>
+__global__ __launch_bounds__(kThreads) void upsample2_kernel(UpArgs a) {
+  typedef typename AccOf<TO>::type A;
+  if (solve_done(a.done)) return;
+  ChunkPos c;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (!chunk_pos<VZ>(cid, a.n1, a.n2, a.n3, c)) return;
+  const int b = blockIdx.y;
+  const int m1 = a.n1 >> 1, m2 = a.n2 >> 1, m3 = a.n3 >> 1;
+  const long Vf = (long)a.n1 * a.n2 * a.n3, Vc = Vf / 8;
+  const TI* __restrict__ in = (const TI*)a.in + b * Vc;
+  int i0, i1, j0, j1;
+  float li, lj;
+  up_axis(c.i, m1, a.nearest, i0, i1, li);
+  up_axis(c.j, m2, a.nearest, j0, j1, lj);
+  const TI* r00 = in + ((long)i0 * m2 + j0) * m3;
+  const TI* r01 = in + ((long)i0 * m2 + j1) * m3;
+  const TI* r10 = in + ((long)i1 * m2 + j0) * m3;
+  const TI* r11 = in + ((long)i1 * m2 + j1) * m3;
+  const A hi0 = A(1) - (A)li, hi1 = (A)li, hj0 = A(1) - (A)lj, hj1 = (A)lj;
+  A out[VZ];
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) {
+    int k0, k1;
+    float lk;
+    up_axis(c.k0 + t, m3, a.nearest, k0, k1, lk);
+    const A w0 = A(1) - (A)lk, w1 = (A)lk;
+    const A v0 = hj0 * (w0 * (A)r00[k0] + w1 * (A)r00[k1]) + hj1 * (w0 * (A)r01[k0] + w1 * (A)r01[k1]);
+    const A v1 = hj0 * (w0 * (A)r10[k0] + w1 * (A)r10[k1]) + hj1 * (w0 * (A)r11[k0] + w1 * (A)r11[k1]);
+    out[t] = (hi0 * v0 + hi1 * v1) * (A)a.scale;
+  }
+  const long off = b * Vf + c.v0;
+  if (a.prev) {
+    const A c0 = *(const A*)a.c0, c1 = *(const A*)a.c1;
+    A pv[VZ];
+    load_vec<VZ>((const TP*)a.prev + off, pv);
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) out[t] = c0 * pv[t] + c1 * out[t];
+  }
+  store_vec<VZ>((TO*)a.out + off, out);
+}
+
+// Adjoint of upsample2 (its transpose as a linear map), as a gather over the fine
+// cells that read each coarse cell.  (n1, n2, n3) are the COARSE dims.
+__device__ __forceinline__ float up_weight(int f, int cidx, int nc, bool nearest) {
+  int i0, i1;
+  float lam;
+  up_axis(f, nc, nearest, i0, i1, lam);
+  float w = 0.f;
+  if (i0 == cidx) w += 1.f - lam;
+  if (i1 == cidx) w += lam;
+  return w;
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kThreads) void upsample2_adjoint_kernel(const TI* __restrict__ gin,
+                                                                     TO* __restrict__ gout, int B,
+                                                                     int n1, int n2, int n3,
+                                                                     int nearest, double scale) {
+  typedef typename AccOf<TO>::type A;
+  const long Vc = (long)n1 * n2 * n3;
+  const long id = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (id >= Vc) return;
+  const int b = blockIdx.y;
+  const int k = (int)(id % n3);
+  const int j = (int)((id / n3) % n2);
+  const int i = (int)(id / ((long)n2 * n3));
+  const int f1 = 2 * n1, f2 = 2 * n2, f3 = 2 * n3;
+  const TI* __restrict__ g = gin + b * Vc * 8;
+  A acc = A(0);
+  for (int fi = max(2 * i - 1, 0); fi <= min(2 * i + 2, f1 - 1); ++fi) {
+    const A wi = (A)up_weight(fi, i, n1, nearest);
+    if (wi == A(0)) continue;
+    for (int fj = max(2 * j - 1, 0); fj <= min(2 * j + 2, f2 - 1); ++fj) {
+      const A wj = (A)up_weight(fj, j, n2, nearest);
+      if (wj == A(0)) continue;
+      const TI* row = g + ((long)fi * f2 + fj) * f3;
+      A s = A(0);
+      for (int fk = max(2 * k - 1, 0); fk <= min(2 * k + 2, f3 - 1); ++fk) {
+        const A wk = (A)up_weight(fk, k, n3, nearest);
+        s += wk * (A)row[fk];
+      }
+      acc += wi * wj * s;
+    }
+  }
+  gout[b * Vc + id] = (TO)(acc * (A)scale);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static int pick_vz(int n3, int maxvz) {
+  for (int vz = maxvz; vz > 1; vz >>= 1)
+    if (n3 % vz == 0) return vz;
+  return 1;
+}
+
+static inline dim3 grid1(long items, int y = 1) {
+  return dim3((unsigned)((items + kThreads - 1) / kThreads), (unsigned)y, 1);
+}
+
+template <typename TI, typename TK>
+static void launch_weights_t(const void* img, const void* W, const void* b, void* K, int C, int n1,
+                             int n2, int n3, bool trans, hipStream_t st) {
+  typedef typename AccOf<TK>::type A;
+  const int maxvz = (C == 1) ? 4 : 2;
+  const int vz = pick_vz(n3, maxvz);
+  const long items = (long)n1 * n2 * (n3 / vz);
+#define MLPCG_SW(CC, VZZ)                                                                      \
+  if (C == CC && vz == VZZ) {                                                                  \
+    if (trans)                                                                                 \
+      stencil_weights_t_kernel<TI, TK, CC, VZZ><<<grid1(items), kThreads, 0, st>>>(           \
+          (const TI*)img, (const A*)W, (const A*)b, (TK*)K, n1, n2, n3);                       \
+    else                                                                                       \
+      stencil_weights_kernel<TI, TK, CC, VZZ><<<grid1(items), kThreads, 0, st>>>(             \
+          (const TI*)img, (const A*)W, (const A*)b, (TK*)K, n1, n2, n3);                       \
+    return;                                                                                    \
+  }
+  MLPCG_SW(1, 4) MLPCG_SW(1, 2) MLPCG_SW(1, 1)
+  MLPCG_SW(2, 2) MLPCG_SW(2, 1)
+  MLPCG_SW(3, 2) MLPCG_SW(3, 1)
+#undef MLPCG_SW
+}
+
+int launch_stencil_weights(int dtype, const void* img, const void* W, const void* b, void* K, int C,
+                           int n1, int n2, int n3, bool trans, hipStream_t st) {
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "stencil_weights: C must be 1, 2 or 3 (got %d)", C);
+  switch (dtype) {
+    case kF64: launch_weights_t<double, double>(img, W, b, K, C, n1, n2, n3, trans, st); break;
+    case kF32: launch_weights_t<float, float>(img, W, b, K, C, n1, n2, n3, trans, st); break;
+    case kF16: launch_weights_t<f16, f16>(img, W, b, K, C, n1, n2, n3, trans, st); break;
+    case kBF16: launch_weights_t<bf16, bf16>(img, W, b, K, C, n1, n2, n3, trans, st); break;
+    default: MLPCG_REQUIRE(false, "stencil_weights: bad dtype %d", dtype);
+  }
+  return check_launch("stencil_weights");
+}
+
+int launch_stencil_weights2(int tin, int tk, const void* img, const void* W, const void* b, void* K,
+                            int C, int n1, int n2, int n3, bool trans, hipStream_t st) {
+  if (tin == tk) return launch_stencil_weights(tk, img, W, b, K, C, n1, n2, n3, trans, st);
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "stencil_weights: C must be 1, 2 or 3 (got %d)", C);
+  MLPCG_REQUIRE(tin == kF32, "stencil_weights: image type %d with stencil type %d unsupported", tin, tk);
+  switch (tk) {
+    case kF16: launch_weights_t<float, f16>(img, W, b, K, C, n1, n2, n3, trans, st); break;
+    case kBF16: launch_weights_t<float, bf16>(img, W, b, K, C, n1, n2, n3, trans, st); break;
+    default: MLPCG_REQUIRE(false, "stencil_weights: bad stencil type %d", tk);
+  }
+  return check_launch("stencil_weights");
+}
+
+template <typename TK, typename TX, typename TY, typename TP>
+static void launch_apply_t(const ApplyArgs& a, int vz, hipStream_t st) {
+  const long items = (long)a.n1 * a.n2 * (a.n3 / vz);
+  const int bb = a.B == 1 ? 1 : 4;
+  const int gy = (a.B + bb - 1) / bb;
+#define MLPCG_AP(VZZ, BBB)                                                                 \
+  if (vz == VZZ && bb == BBB) {                                                            \
+    sm_apply_kernel<TK, TX, TY, TP, VZZ, BBB><<<grid1(items, gy), kThreads, 0, st>>>(a);  \
+    return;                                                                                \
+  }
+  MLPCG_AP(8, 1) MLPCG_AP(4, 1) MLPCG_AP(2, 1) MLPCG_AP(1, 1)
+  MLPCG_AP(8, 4) MLPCG_AP(4, 4) MLPCG_AP(2, 4) MLPCG_AP(1, 4)
+#undef MLPCG_AP
+}
+
+int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& a, hipStream_t st) {
+  MLPCG_REQUIRE(a.B >= 1 && a.n1 >= 1 && a.n2 >= 1 && a.n3 >= 1, "sm_apply: bad sizes");
+  const int vz = pick_vz(a.n3, tk == kF64 ? 4 : 8);
+  if (tk == kF64) {
+    MLPCG_REQUIRE(tx == kF64 && ty == kF64, "sm_apply: f64 stencil needs f64 vectors");
+    launch_apply_t<double, double, double, double>(a, vz, st);
+  } else if (tk == kF32) {
+    MLPCG_REQUIRE(tx == kF32 && ty == kF32, "sm_apply: f32 stencil needs f32 vectors");
+    launch_apply_t<float, float, float, float>(a, vz, st);
+  } else if (tk == kBF16 || tk == kF16) {
+    const bool ok = (tx == tk && ty == tk) || (tx == kF32 && ty == tk) || (tx == tk && ty == kF32);
+    MLPCG_REQUIRE(ok, "sm_apply: unsupported vector types (%d,%d) for stencil type %d", tx, ty, tk);
+    if (tk == kBF16) {
+      if (tx == tk && ty == tk) launch_apply_t<bf16, bf16, bf16, bf16>(a, vz, st);
+      else if (tx == kF32) launch_apply_t<bf16, float, bf16, bf16>(a, vz, st);
+      else launch_apply_t<bf16, bf16, float, bf16>(a, vz, st);
+    } else {
+      if (tx == tk && ty == tk) launch_apply_t<f16, f16, f16, f16>(a, vz, st);
+      else if (tx == kF32) launch_apply_t<f16, float, f16, f16>(a, vz, st);
+      else launch_apply_t<f16, f16, float, f16>(a, vz, st);
+    }
+  } else {
+    MLPCG_REQUIRE(false, "sm_apply: bad dtype %d", tk);
+  }
+  return check_launch("sm_apply");
+}
+
+template <typename T>
+static void launch_pool_t(const void* in, void* out, int B, int n1, int n2, int n3, const int* done,
+                          hipStream_t st) {
+  const int vz = pick_vz(n3, 4);
+  const long items = (long)n1 * n2 * (n3 / vz);
+  if (vz == 4)
+    avgpool2_kernel<T, T, 4><<<grid1(items, B), kThreads, 0, st>>>((const T*)in, (T*)out, B, n1, n2, n3, done);
+  else if (vz == 2)
+    avgpool2_kernel<T, T, 2><<<grid1(items, B), kThreads, 0, st>>>((const T*)in, (T*)out, B, n1, n2, n3, done);
+  else
+    avgpool2_kernel<T, T, 1><<<grid1(items, B), kThreads, 0, st>>>((const T*)in, (T*)out, B, n1, n2, n3, done);
+}
+
+int launch_avgpool2(int dtype, const void* in, void* out, int B, int n1, int n2, int n3,
+                    const int* done, hipStream_t st) {
+  MLPCG_REQUIRE(n1 % 2 == 0 && n2 % 2 == 0 && n3 % 2 == 0,
+                "avgpool2: grid (%d,%d,%d) must be even in every dimension", n1, n2, n3);
+  const int c1 = n1 / 2, c2 = n2 / 2, c3 = n3 / 2;
+  switch (dtype) {
+    case kF64: launch_pool_t<double>(in, out, B, c1, c2, c3, done, st); break;
+    case kF32: launch_pool_t<float>(in, out, B, c1, c2, c3, done, st); break;
+    case kF16: launch_pool_t<f16>(in, out, B, c1, c2, c3, done, st); break;
+    case kBF16: launch_pool_t<bf16>(in, out, B, c1, c2, c3, done, st); break;
+    default: MLPCG_REQUIRE(false, "avgpool2: bad dtype %d", dtype);
+  }
+  return check_launch("avgpool2");
+}
+
+template <typename TI, typename TO>
+static void launch_up_t(const UpArgs& a, hipStream_t st) {
+  const int vz = pick_vz(a.n3, 4);
+  const long items = (long)a.n1 * a.n2 * (a.n3 / vz);
+  if (vz == 4) upsample2_kernel<TI, TO, TO, 4><<<grid1(items, a.B), kThreads, 0, st>>>(a);
+  else if (vz == 2) upsample2_kernel<TI, TO, TO, 2><<<grid1(items, a.B), kThreads, 0, st>>>(a);
+  else upsample2_kernel<TI, TO, TO, 1><<<grid1(items, a.B), kThreads, 0, st>>>(a);
+}
+
+int launch_upsample2(int dtype, const UpArgs& a, hipStream_t st) {
+  MLPCG_REQUIRE(a.n1 % 2 == 0 && a.n2 % 2 == 0 && a.n3 % 2 == 0,
+                "upsample2: fine grid (%d,%d,%d) must be even", a.n1, a.n2, a.n3);
+  switch (dtype) {
+    case kF64: launch_up_t<double, double>(a, st); break;
+    case kF32: launch_up_t<float, float>(a, st); break;
+    case kF16: launch_up_t<f16, f16>(a, st); break;
+    case kBF16: launch_up_t<bf16, bf16>(a, st); break;
+    default: MLPCG_REQUIRE(false, "upsample2: bad dtype %d", dtype);
+  }
+  return check_launch("upsample2");
+}
+
+int launch_upsample2_adjoint(int dtype, const void* gin, void* gout, int B, int n1, int n2, int n3,
+                             int nearest, double scale, hipStream_t st) {
+  const long items = (long)n1 * n2 * n3;
+  switch (dtype) {
+    case kF64:
+      upsample2_adjoint_kernel<double, double><<<grid1(items, B), kThreads, 0, st>>>(
+          (const double*)gin, (double*)gout, B, n1, n2, n3, nearest, scale);
+      break;
+    case kF32:
+      upsample2_adjoint_kernel<float, float><<<grid1(items, B), kThreads, 0, st>>>(
+          (const float*)gin, (float*)gout, B, n1, n2, n3, nearest, scale);
+      break;
+    case kF16:
+      upsample2_adjoint_kernel<f16, f16><<<grid1(items, B), kThreads, 0, st>>>(
+          (const f16*)gin, (f16*)gout, B, n1, n2, n3, nearest, scale);
+      break;
+    case kBF16:
+      upsample2_adjoint_kernel<bf16, bf16><<<grid1(items, B), kThreads, 0, st>>>(
+          (const bf16*)gin, (bf16*)gout, B, n1, n2, n3, nearest, scale);
+      break;
+    default: MLPCG_REQUIRE(false, "upsample2_adjoint: bad dtype %d", dtype);
+  }
+  return check_launch("upsample2_adjoint");
+}
+
+}  // namespace mlpcg

@@ -1,0 +1,233 @@
+"""Tensor-level wrappers of the C ABI (one function per entry point family).
+
+Shapes follow the reference: image (C, N1, N2, N3); x (B, 1, N1, N2, N3) or any
+tensor whose last three dims are the grid.  Weights are (27, C, 3, 3, 3) + (27,) for SM
+blocks and (1, C, 3, 3, 3) + (1,) for SM linear, passed in the accumulate type (float64
+for float64 data, float32 otherwise).  All work is queued on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import lib, check, ptr, stream_ptr, code_of, require_cuda, ACC_DTYPE
+
+
+def _grid(t):
+    n1, n2, n3 = t.shape[-3:]
+    return int(n1), int(n2), int(n3)
+
+
+def _batch(x):
+    n1, n2, n3 = _grid(x)
+    return x.numel() // (n1 * n2 * n3)
+
+
+def _acc(dtype):
+    return ACC_DTYPE[code_of(dtype)]
+
+
+def _params(weight, bias, dtype):
+    acc = _acc(dtype)
+    return weight.detach().to(acc).contiguous(), bias.detach().to(acc).contiguous()
+
+
+def workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=device)
+
+
+# ------------------------------------------------------------------ SM block / transpose
+
+def _block_forward(fn, image, x, weight, bias):
+    require_cuda(image, x)
+    dt = code_of(x.dtype)
+    image = image.to(x.dtype).contiguous()
+    W, b = _params(weight, bias, x.dtype)
+    n1, n2, n3 = _grid(x)
+    B = _batch(x)
+    Cimg = image.shape[0]
+    y = torch.empty_like(x)
+    ws = workspace(lib.mlpcg_sm_block_workspace_size(Cimg, B, n1, n2, n3, dt), x.device)
+    check(fn(ptr(image), ptr(x), ptr(W), ptr(b), ptr(y), ptr(ws), Cimg, B, n1, n2, n3, dt,
+             stream_ptr(x.device)), fn.__name__)
+    return y
+
+
+def _block_backward(fn, g, image, x, weight, bias):
+    require_cuda(g, image, x)
+    dt = code_of(x.dtype)
+    g = g.to(x.dtype).contiguous()
+    image = image.to(x.dtype).contiguous()
+    W, b = _params(weight, bias, x.dtype)
+    n1, n2, n3 = _grid(x)
+    B = _batch(x)
+    Cimg = image.shape[0]
+    gx = torch.empty_like(x)
+    gW = torch.empty_like(W)
+    gb = torch.empty_like(b)
+    ws = workspace(lib.mlpcg_sm_block_workspace_size(Cimg, B, n1, n2, n3, dt), x.device)
+    check(fn(ptr(g), ptr(image), ptr(x), ptr(W), ptr(b), ptr(gx), ptr(gW), ptr(gb), ptr(ws), Cimg,
+             B, n1, n2, n3, dt, stream_ptr(x.device)), fn.__name__)
+    return gx, gW.to(weight.dtype), gb.to(bias.dtype)
+
+
+def sm_block_forward(image, x, weight, bias):
+    return _block_forward(lib.mlpcg_sm_block_forward, image, x, weight, bias)
+
+
+def sm_block_inference(image, x, weight, bias):
+    return _block_forward(lib.mlpcg_sm_block_inference, image, x, weight, bias)
+
+
+def sm_block_backward(g, image, x, weight, bias):
+    return _block_backward(lib.mlpcg_sm_block_backward, g, image, x, weight, bias)
+
+
+def sm_block_trans_forward(image, x, weight, bias):
+    return _block_forward(lib.mlpcg_sm_block_trans_forward, image, x, weight, bias)
+
+
+def sm_block_trans_inference(image, x, weight, bias):
+    return _block_forward(lib.mlpcg_sm_block_trans_inference, image, x, weight, bias)
+
+
+def sm_block_trans_backward(g, image, x, weight, bias):
+    return _block_backward(lib.mlpcg_sm_block_trans_backward, g, image, x, weight, bias)
+
+
+# ------------------------------------------------------------------ SM linear
+
+def sm_linear(image, weight, bias):
+    """-> (z of shape (1,), ybar of shape (1, C, 3, 3, 3)), accumulate type."""
+    require_cuda(image)
+    dt = code_of(image.dtype)
+    acc = _acc(image.dtype)
+    image = image.contiguous()
+    w, b = _params(weight, bias, image.dtype)
+    Cimg = image.shape[0]
+    n1, n2, n3 = _grid(image)
+    z = torch.empty(1, dtype=acc, device=image.device)
+    ybar = torch.empty(1, Cimg, 3, 3, 3, dtype=acc, device=image.device)
+    ws = workspace(lib.mlpcg_sm_linear_workspace_size(Cimg, n1, n2, n3, dt), image.device)
+    check(lib.mlpcg_sm_linear_forward(ptr(image), ptr(w), ptr(b), ptr(z), ptr(ybar), ptr(ws), Cimg,
+                                      n1, n2, n3, dt, stream_ptr(image.device)), "sm_linear")
+    return z, ybar
+
+
+def sm_linear_backward(g, ybar):
+    require_cuda(g, ybar)
+    dt = code_of(ybar.dtype)
+    g = g.to(ybar.dtype).reshape(1).contiguous()
+    gw = torch.empty_like(ybar)
+    gb = torch.empty(1, dtype=ybar.dtype, device=ybar.device)
+    check(lib.mlpcg_sm_linear_backward(ptr(g), ptr(ybar), ptr(gw), ptr(gb), ybar.shape[1], dt,
+                                       stream_ptr(ybar.device)), "sm_linear_backward")
+    return gw, gb
+
+
+# ------------------------------------------------------------------ streamed-stencil parts
+
+def stencil_weights(image, weight, bias, dtype=None, transposed=False):
+    """K (27, N1, N2, N3) in `dtype` (default: image dtype)."""
+    require_cuda(image)
+    dtype = dtype or image.dtype
+    image = image.contiguous()
+    if image.dtype != dtype and image.dtype != torch.float32:
+        image = image.to(dtype)
+    W, b = _params(weight, bias, dtype)
+    n1, n2, n3 = _grid(image)
+    K = torch.empty(27, n1, n2, n3, dtype=dtype, device=image.device)
+    check(lib.mlpcg_stencil_weights(ptr(image), code_of(image.dtype), ptr(W), ptr(b), ptr(K),
+                                    code_of(dtype), int(transposed), image.shape[0], n1, n2, n3,
+                                    stream_ptr(image.device)), "stencil_weights")
+    return K
+
+
+def sm_apply(K, x, out_dtype=None, prev=None, c0=None, c1=None, mask_a=None, mask_b=None,
+             in_scale=None):
+    require_cuda(K, x, prev, c0, c1, mask_a, mask_b, in_scale)
+    out_dtype = out_dtype or x.dtype
+    n1, n2, n3 = _grid(x)
+    y = torch.empty(x.shape, dtype=out_dtype, device=x.device)
+    check(lib.mlpcg_sm_apply(ptr(K), code_of(K.dtype), ptr(x), code_of(x.dtype), ptr(y),
+                             code_of(out_dtype), ptr(prev), ptr(c0), ptr(c1), ptr(mask_a),
+                             ptr(mask_b), ptr(in_scale), _batch(x), n1, n2, n3,
+                             stream_ptr(x.device)), "sm_apply")
+    return y
+
+
+def avgpool2(x):
+    require_cuda(x)
+    n1, n2, n3 = _grid(x)
+    y = torch.empty(*x.shape[:-3], n1 // 2, n2 // 2, n3 // 2, dtype=x.dtype, device=x.device)
+    check(lib.mlpcg_avgpool2(ptr(x), ptr(y), _batch(x), n1, n2, n3, code_of(x.dtype),
+                             stream_ptr(x.device)), "avgpool2")
+    return y
+
+
+def upsample2(x, mode="trilinear", scale=1.0, prev=None, c0=None, c1=None):
+    require_cuda(x, prev, c0, c1)
+    m1, m2, m3 = _grid(x)
+    y = torch.empty(*x.shape[:-3], 2 * m1, 2 * m2, 2 * m3, dtype=x.dtype, device=x.device)
+    check(lib.mlpcg_upsample2(ptr(x), ptr(y), _batch(x), 2 * m1, 2 * m2, 2 * m3,
+                              int(mode == "nearest"), float(scale), ptr(prev), ptr(c0), ptr(c1),
+                              code_of(x.dtype), stream_ptr(x.device)), "upsample2")
+    return y
+
+
+def upsample2_adjoint(g, mode="trilinear", scale=1.0):
+    require_cuda(g)
+    f1, f2, f3 = _grid(g)
+    y = torch.empty(*g.shape[:-3], f1 // 2, f2 // 2, f3 // 2, dtype=g.dtype, device=g.device)
+    check(lib.mlpcg_upsample2_adjoint(ptr(g), ptr(y), _batch(g), f1 // 2, f2 // 2, f3 // 2,
+                                      int(mode == "nearest"), float(scale), code_of(g.dtype),
+                                      stream_ptr(g.device)), "upsample2_adjoint")
+    return y
+
+
+# ------------------------------------------------------------------ Laplacian / dots
+
+def flags_code(flags):
+    """flags: int8 tensor (N1, N2, N3) on the GPU -> uint8 code tensor."""
+    require_cuda(flags)
+    flags = flags.to(torch.int8).contiguous()
+    n1, n2, n3 = _grid(flags)
+    code = torch.empty(flags.shape, dtype=torch.uint8, device=flags.device)
+    check(lib.mlpcg_flags_code(ptr(flags), ptr(code), n1, n2, n3, stream_ptr(flags.device)),
+          "flags_code")
+    return code
+
+
+def spmv_flags(code, x):
+    require_cuda(code, x)
+    n1, n2, n3 = _grid(code)
+    y = torch.empty_like(x)
+    check(lib.mlpcg_spmv_flags(ptr(code), ptr(x), ptr(y), n1, n2, n3, stream_ptr(x.device)),
+          "spmv_flags")
+    return y
+
+
+def spmv_csr(rowptr, col, val, x):
+    require_cuda(rowptr, col, val, x)
+    y = torch.empty_like(x)
+    check(lib.mlpcg_spmv_csr(ptr(rowptr), ptr(col), ptr(val), ptr(x), ptr(y), x.numel(),
+                             code_of(val.dtype), stream_ptr(x.device)), "spmv_csr")
+    return y
+
+
+_dot_ws = {}
+
+
+def dot(x, y):
+    """FP64-accumulated dot of two float32 vectors -> 0-dim float64 tensor (device)."""
+    require_cuda(x, y)
+    dev = x.device
+    ws = _dot_ws.get(dev)
+    if ws is None:
+        ws = torch.zeros(lib.mlpcg_dot_workspace_size(), dtype=torch.uint8, device=dev)
+        _dot_ws[dev] = ws
+    out = torch.empty((), dtype=torch.float64, device=dev)
+    check(lib.mlpcg_dot(ptr(x), ptr(y), x.numel(), ptr(out), ptr(ws), stream_ptr(dev)), "dot")
+    return out

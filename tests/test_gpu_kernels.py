@@ -1,0 +1,221 @@
+"""GPU parity of the individual kernels (through the C ABI) against the FP64 oracle.
+
+Tolerances: FP64 kernels 1e-11 relative to the output scale (same sums, different
+order); FP32 1e-5 relative (the stated preconditioner bar); BF16/FP16 storage only
+for the stencil weights, at their rounding (2^-8 / 2^-11 relative)."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from oracle import model as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand(shape, seed, dtype=torch.float64):
+    g = torch.Generator().manual_seed(seed)
+    return (torch.rand(*shape, generator=g, dtype=torch.float64) * 2 - 1).to(dtype)
+
+
+def _rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).abs().max() / max(b.abs().max(), 1e-300))
+
+
+SHAPES = [(8, 8, 8), (6, 10, 12), (4, 6, 5), (5, 3, 16)]
+
+
+@pytest.mark.parametrize("C", [1, 2, 3])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_stencil_weights_f64(dev, C, shape):
+    from mlpcg_amd import kernels as K
+    img, W, b = _rand((C, *shape), 1), _rand((27, C, 3, 3, 3), 2), _rand((27,), 3)
+    Kg = K.stencil_weights(img.to(dev), W.to(dev), b.to(dev))
+    Kr = M.stencil_weights(img, W, b)
+    assert _rel(Kg, Kr) < 1e-12
+
+
+@pytest.mark.parametrize("C", [1, 3])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_stencil_weights_transposed_f64(dev, C, shape):
+    from mlpcg_amd import kernels as K
+    img, W, b = _rand((C, *shape), 4), _rand((27, C, 3, 3, 3), 5), _rand((27,), 6)
+    Kt = K.stencil_weights(img.to(dev), W.to(dev), b.to(dev), transposed=True).cpu()
+    Kr = M.stencil_weights(img, W, b)
+    ref = torch.zeros_like(Kr)
+    for q, d in enumerate(M.TAPS):
+        ref[q] = M._shifted(Kr[26 - q], d)
+    assert _rel(Kt, ref) < 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-6), (torch.bfloat16, 2 ** -8),
+                                       (torch.float16, 2 ** -10)])
+def test_stencil_weights_storage_types(dev, dtype, tol):
+    from mlpcg_amd import kernels as K
+    C, shape = 1, (16, 8, 32)
+    img, W, b = _rand((C, *shape), 7).abs(), _rand((27, C, 3, 3, 3), 8), _rand((27,), 9)
+    Kg = K.stencil_weights(img.float().to(dev), W.float().to(dev), b.float().to(dev), dtype=dtype)
+    assert Kg.dtype == dtype
+    Kr = M.stencil_weights(img.float().double(), W.float().double(), b.float().double())
+    err = (Kg.double().cpu() - Kr).abs() / Kr.abs().clamp(min=1e-3)
+    assert float(err.max()) < tol
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("C,B,shape", [(1, 1, (8, 8, 8)), (3, 3, (6, 10, 12)), (2, 5, (4, 6, 5)),
+                                       (1, 2, (5, 3, 16))])
+def test_sm_block_forward_f64(dev, trans, C, B, shape):
+    from mlpcg_amd import kernels as K
+    img, x = _rand((C, *shape), 10), _rand((B, 1, *shape), 11)
+    W, b = _rand((27, C, 3, 3, 3), 12), _rand((27,), 13)
+    f = K.sm_block_trans_forward if trans else K.sm_block_forward
+    y = f(img.to(dev), x.to(dev), W.to(dev), b.to(dev))
+    yr = (M.sm_block_trans if trans else M.sm_block)(img, x, W, b)
+    assert _rel(y, yr) < 1e-12
+    # inference entry point: same result
+    fi = K.sm_block_trans_inference if trans else K.sm_block_inference
+    assert torch.equal(fi(img.to(dev), x.to(dev), W.to(dev), b.to(dev)), y)
+
+
+@pytest.mark.parametrize("trans", [False, True])
+def test_sm_block_forward_f32(dev, trans):
+    from mlpcg_amd import kernels as K
+    C, B, shape = 3, 2, (16, 16, 16)
+    img = (_rand((C, *shape), 14) > 0).float()
+    x = _rand((B, 1, *shape), 15).float()
+    W, b = _rand((27, C, 3, 3, 3), 16).float() / 9, _rand((27,), 17).float() / 9
+    f = K.sm_block_trans_forward if trans else K.sm_block_forward
+    y = f(img.to(dev), x.to(dev), W.to(dev), b.to(dev))
+    yr = (M.sm_block_trans if trans else M.sm_block)(img.double(), x.double(), W.double(), b.double())
+    assert _rel(y, yr) < 1e-5
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("C,B,shape", [(1, 2, (6, 5, 8)), (3, 1, (4, 6, 4)), (2, 3, (5, 4, 6))])
+def test_sm_block_backward_f64(dev, trans, C, B, shape):
+    """Gradients vs torch autograd of the FP64 oracle."""
+    from mlpcg_amd import kernels as K
+    img, x, g = _rand((C, *shape), 18), _rand((B, 1, *shape), 19), _rand((B, 1, *shape), 20)
+    W, b = _rand((27, C, 3, 3, 3), 21), _rand((27,), 22)
+    f = K.sm_block_trans_backward if trans else K.sm_block_backward
+    gx, gW, gb = f(g.to(dev), img.to(dev), x.to(dev), W.to(dev), b.to(dev))
+    xr, Wr, br = x.clone().requires_grad_(), W.clone().requires_grad_(), b.clone().requires_grad_()
+    y = (M.sm_block_trans if trans else M.sm_block)(img, xr, Wr, br)
+    (y * g).sum().backward()
+    assert _rel(gx, xr.grad) < 1e-11
+    assert _rel(gW, Wr.grad) < 1e-11
+    assert _rel(gb, br.grad) < 1e-11
+
+
+@pytest.mark.parametrize("trans", [False, True])
+def test_sm_block_backward_finite_differences_f64(dev, trans):
+    """Central differences of the GPU forward in FP64 on a 16^3 grid (to 1e-6)."""
+    from mlpcg_amd import kernels as K
+    C, B, shape = 1, 2, (16, 16, 16)
+    img = _rand((C, *shape), 23).abs().to(dev)
+    x, R = _rand((B, 1, *shape), 24).to(dev), _rand((B, 1, *shape), 25).to(dev)
+    W, b = (_rand((27, C, 3, 3, 3), 26) / 9).to(dev), (_rand((27,), 27) / 9).to(dev)
+    fwd = K.sm_block_trans_forward if trans else K.sm_block_forward
+    bwd = K.sm_block_trans_backward if trans else K.sm_block_backward
+    gx, gW, gb = bwd(R, img, x, W, b)
+    L = lambda x_, W_, b_: float((fwd(img, x_, W_, b_) * R).sum())
+    h = 1e-4
+    for seed in range(3):
+        dW, db, dx = _rand(W.shape, 100 + seed).to(dev), _rand(b.shape, 200 + seed).to(dev), \
+            _rand(x.shape, 300 + seed).to(dev)
+        fd = (L(x, W + h * dW, b) - L(x, W - h * dW, b)) / (2 * h)
+        an = float((gW * dW).sum())
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an))
+        fd = (L(x, W, b + h * db) - L(x, W, b - h * db)) / (2 * h)
+        assert abs(fd - float((gb * db).sum())) <= 1e-6 * max(1.0, abs(fd))
+        fd = (L(x + h * dx, W, b) - L(x - h * dx, W, b)) / (2 * h)
+        assert abs(fd - float((gx * dx).sum())) <= 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("C,shape", [(1, (8, 8, 8)), (3, (6, 10, 12)), (2, (1, 4, 5)), (1, (16, 16, 32))])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_sm_linear(dev, C, shape, dtype):
+    from mlpcg_amd import kernels as K
+    img, w, b = _rand((C, *shape), 28).to(dtype), _rand((1, C, 3, 3, 3), 29).to(dtype), _rand((1,), 30).to(dtype)
+    z, ybar = K.sm_linear(img.to(dev), w.to(dev), b.to(dev))
+    zr, ybr = M.sm_linear(img.double(), w.double(), b.double())
+    tol = 1e-12 if dtype == torch.float64 else 1e-6
+    assert _rel(ybar, ybr) < tol
+    assert abs(float(z) - float(zr)) < tol * max(1.0, float(zr.abs()))
+    g = torch.tensor([1.7], dtype=z.dtype, device=dev)
+    gw, gb = K.sm_linear_backward(g, ybar)
+    assert torch.allclose(gw, 1.7 * ybar) and float(gb) == pytest.approx(1.7)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (4, 6, 10), (2, 2, 2)])
+def test_avgpool2(dev, dtype, shape):
+    from mlpcg_amd import kernels as K
+    x = _rand((3, 1, *shape), 31).to(dtype)
+    y = K.avgpool2(x.to(dev))
+    yr = F.avg_pool3d(x.double(), (2, 2, 2))
+    tol = {torch.float64: 1e-15, torch.float32: 1e-7, torch.bfloat16: 2 ** -8, torch.float16: 2 ** -11}[dtype]
+    assert _rel(y, yr) < tol
+
+
+@pytest.mark.parametrize("mode", ["trilinear", "nearest"])
+@pytest.mark.parametrize("shape", [(4, 4, 4), (2, 3, 5), (1, 1, 1), (8, 4, 2)])
+def test_upsample2_f64(dev, mode, shape):
+    from mlpcg_amd import kernels as K
+    x = _rand((2, 1, *shape), 32)
+    y = K.upsample2(x.to(dev), mode, 1.0)
+    yr = F.interpolate(x, scale_factor=2, mode=mode) if mode != "nearest" else F.interpolate(x, scale_factor=2)
+    assert _rel(y, yr) < 1e-14
+    # fused axpby form and scale
+    prev = _rand(y.shape, 33).to(dev)
+    c0 = torch.tensor([0.3], dtype=torch.float64, device=dev)
+    c1 = torch.tensor([-1.5], dtype=torch.float64, device=dev)
+    y2 = K.upsample2(x.to(dev), mode, 0.125, prev=prev, c0=c0, c1=c1)
+    assert _rel(y2, 0.3 * prev - 1.5 * 0.125 * y) < 1e-14
+
+
+@pytest.mark.parametrize("mode", ["trilinear", "nearest"])
+@pytest.mark.parametrize("shape", [(4, 4, 4), (2, 3, 5), (1, 2, 1)])
+def test_upsample2_adjoint_f64(dev, mode, shape):
+    """<U x, y> = <x, U^T y>."""
+    from mlpcg_amd import kernels as K
+    x = _rand((2, 1, *shape), 34).to(dev)
+    yf = _rand((2, 1, *(2 * s for s in shape)), 35).to(dev)
+    lhs = float((K.upsample2(x, mode, 0.5) * yf).sum())
+    rhs = float((x * K.upsample2_adjoint(yf, mode, 0.5)).sum())
+    assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(lhs))
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-6), (torch.bfloat16, 2 ** -7)])
+def test_sm_apply_epilogue(dev, dtype, tol):
+    """y = c0 * prev + c1 * (scale * stencil), masked before and after."""
+    from mlpcg_amd import kernels as K
+    shape = (8, 8, 16)
+    img, W, b = _rand((1, *shape), 36).abs(), _rand((27, 1, 3, 3, 3), 37) / 9, _rand((27,), 38) / 9
+    x = _rand((1, 1, *shape), 39)
+    prev = _rand((1, 1, *shape), 40)
+    ma = (_rand(shape, 41) > -0.5).to(torch.uint8)
+    mb = (_rand(shape, 42) > -0.5).to(torch.uint8)
+    Kd = K.stencil_weights(img.float().to(dev), W.float().to(dev), b.float().to(dev), dtype=dtype)
+    scale = torch.tensor(0.25, dtype=torch.float64, device=dev)
+    c0 = torch.tensor([0.7], device=dev)
+    c1 = torch.tensor([-1.3], device=dev)
+    y = K.sm_apply(Kd, x.float().to(dev), out_dtype=dtype, prev=prev.to(dtype).to(dev), c0=c0, c1=c1,
+                   mask_a=ma.to(dev), mask_b=mb.to(dev), in_scale=scale)
+    ref = 0.25 * M.sm_block(img.float().double(), x.float().double(), W.float().double(), b.float().double())
+    ref = torch.where(ma.bool(), ref, torch.zeros(()))
+    ref = 0.7 * prev.to(dtype).double() - 1.3 * ref
+    ref = torch.where(mb.bool(), ref, torch.zeros(()))
+    assert _rel(y, ref) < tol
+
+
+def test_dot_f64_accumulation(dev):
+    from mlpcg_amd import kernels as K
+    n = 1_000_003
+    x = _rand((n,), 43).float()
+    y = _rand((n,), 44).float()
+    d = float(K.dot(x.to(dev), y.to(dev)))
+    ref = float((x.double() * y.double()).sum())
+    assert abs(d - ref) < 1e-12 * np.sqrt(n)

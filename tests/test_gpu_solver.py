@@ -1,0 +1,134 @@
+"""GPU parity of the Laplacian operators and the native PCG driver against scipy and the
+FP64 solver restatement (oracle/solvers.py)."""
+import numpy as np
+import pytest
+import torch
+
+from mlpcg_amd import scenes as S
+from oracle import model as M
+from oracle import solvers as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(n=32, seed=3):
+    sc = S.scene_spheres(n, seed, num_spheres=3, rmin=3, rmax=7)
+    A = S.build_matrix(sc.flags)
+    b = S.make_rhs(sc.flags, seed)[0]
+    return sc, A, b
+
+
+def test_spmv_flat_and_csr_vs_scipy(dev):
+    from mlpcg_amd import kernels as K
+    from mlpcg_amd.solver import CSRMatrix
+    sc, A, _ = _scene()
+    rng = np.random.default_rng(0)
+    xf = rng.integers(-50, 50, size=A.shape[0]).astype(np.float32)  # integer values: exact
+    ref = A @ xf.astype(np.float64)
+    code = K.flags_code(torch.as_tensor(sc.flags, device=dev))
+    xg = torch.as_tensor(S.expand(xf, sc.flags).ravel(), device=dev)
+    yg = K.spmv_flags(code, xg).cpu().numpy()
+    y_flat = yg[sc.fluid_cells]
+    assert np.array_equal(y_flat, ref.astype(np.float32))
+    assert np.all(yg[np.setdiff1d(np.arange(yg.size), sc.fluid_cells)] == 0)
+    Ad = CSRMatrix(A, dev)
+    y_csr = (Ad @ torch.as_tensor(xf, device=dev)).cpu().numpy()
+    assert np.array_equal(y_csr, y_flat)
+    # code = 1 + diag
+    c = code.cpu().numpy().ravel()[sc.fluid_cells]
+    assert np.array_equal(c.astype(np.float64) - 1, A.diagonal())
+
+
+def _hist_close(h_gpu, h_ref, m, rtol):
+    m = min(m, len(h_gpu), len(h_ref))
+    a, b = np.asarray(h_gpu[:m]), np.asarray(h_ref[:m])
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+@pytest.mark.parametrize("layout", ["flags", "csr"])
+def test_plain_cg_vs_oracle(dev, layout):
+    from mlpcg_amd.solver import FusedPCG
+    sc, A, b = _scene()
+    s = FusedPCG(sc.flags, layout, A=A if layout == "csr" else None, device=dev)
+    bv = s.vector(b)
+    x, info = s.solve(bv, None, method="npcg", max_it=600, tol=1e-6, history=True)
+    _, k_ref, h_ref = O.npcg(b, A, np.zeros_like(b), lambda r: r.copy(), 600, tol=1e-6)
+    assert info["status"] == 0
+    xs = x.cpu().numpy().astype(np.float64)
+    if layout == "flags":
+        xs = xs[sc.fluid_cells]
+    res = np.linalg.norm(b - A @ xs) / np.linalg.norm(b)
+    assert res <= 1.05e-6                            # FP64 check of the returned iterate
+    assert abs(res - info["rel_res"]) < 1e-9 + 1e-3 * res
+    assert abs(info["iters"] - k_ref) <= max(2, k_ref // 20)
+    assert _hist_close(info["history"], h_ref, 30, 1e-3) < 1e-3
+
+
+def test_flags_and_csr_agree(dev):
+    from mlpcg_amd.solver import FusedPCG
+    sc, A, b = _scene(seed=4)
+    s1 = FusedPCG(sc.flags, "flags", device=dev)
+    s2 = FusedPCG(sc.flags, "csr", A=A, device=dev)
+    _, i1 = s1.solve(s1.vector(b), None, max_it=300, tol=1e-5, history=True)
+    _, i2 = s2.solve(s2.vector(b), None, max_it=300, tol=1e-5, history=True)
+    assert abs(i1["iters"] - i2["iters"]) <= 1
+    assert _hist_close(i1["history"], i2["history"], 40, 1e-4) < 1e-4
+
+
+def _model_and_plan(sc, n, dev, dtype=torch.float32, seed=21):
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    P = M.init_params(n, 1, torch.Generator().manual_seed(seed))
+    m = SmallSMModelDn3D(n, 1).to(dev)
+    m.load_state_dict({k: v.float() for k, v in P.items()})
+    img = torch.as_tensor(S.image_channels(sc.flags, 1), device=dev)
+    plan = m.make_preconditioner(img, dtype)
+    Pf = {k: v.float().double() for k, v in P.items()}
+    img64 = img.double().cpu()
+    model_fn = lambda x: M.model_forward(Pf, n, img64, x)
+    return plan, model_fn
+
+
+@pytest.mark.parametrize("method", ["npcg", "npcg_flex", "npsd"])
+@pytest.mark.parametrize("layout", ["flags", "csr"])
+def test_preconditioned_history_vs_oracle(dev, method, layout):
+    """Same residual history as the FP64 loop with the FP64 model (random weights: the
+    comparison is over the first iterations, before FP32 rounding separates them)."""
+    from mlpcg_amd.solver import FusedPCG
+    sc, A, b = _scene(n=32, seed=5)
+    plan, model_fn = _model_and_plan(sc, 3, dev)
+    s = FusedPCG(sc.flags, layout, A=A if layout == "csr" else None, device=dev)
+    it = 8
+    _, info = s.solve(s.vector(b), plan, method=method, max_it=it, tol=1e-12, history=True)
+    predict = O.make_predict(model_fn, sc.fluid_cells, sc.shape)
+    f = {"npcg": O.npcg, "npcg_flex": O.npcg_flex, "npsd": O.npsd}[method]
+    _, k, h_ref = f(b, A, np.zeros_like(b), predict, it, tol=1e-12)
+    assert info["iters"] == k == it
+    assert np.all(np.isfinite(info["history"]))
+    assert _hist_close(info["history"], h_ref, it + 1, 1e-3) < 2e-3
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+def test_low_precision_plan_solve_runs(dev, dtype):
+    from mlpcg_amd.solver import FusedPCG
+    sc, A, b = _scene(n=32, seed=6)
+    plan, _ = _model_and_plan(sc, 3, dev, dtype)
+    s = FusedPCG(sc.flags, "flags", device=dev)
+    x, info = s.solve(s.vector(b), plan, method="npcg", max_it=20, tol=1e-12, history=True)
+    assert info["iters"] == 20 and np.all(np.isfinite(info["history"]))
+    xs = x.cpu().numpy()[sc.fluid_cells].astype(np.float64)
+    res = np.linalg.norm(b - A @ xs) / np.linalg.norm(b)
+    assert abs(res - info["rel_res"]) < 1e-6 + 1e-3 * res
+
+
+def test_solver_reuse_and_x0(dev):
+    """A second solve on the same solver object gives the same answer; a warm start
+    with the solution stops after one iteration."""
+    from mlpcg_amd.solver import FusedPCG
+    sc, A, b = _scene(n=32, seed=7)
+    s = FusedPCG(sc.flags, "flags", device=dev)
+    bv = s.vector(b)
+    x1, i1 = s.solve(bv, None, max_it=400, tol=1e-6)
+    x2, i2 = s.solve(bv, None, max_it=400, tol=1e-6)
+    assert i1["iters"] == i2["iters"] and torch.equal(x1, x2)
+    _, i3 = s.solve(bv, None, x0=x1, max_it=400, tol=1e-5)
+    assert i3["iters"] == 1

@@ -149,6 +149,12 @@ def sm_apply(K, x, out_dtype=None, prev=None, c0=None, c1=None, mask_a=None, mas
              in_scale=None):
     require_cuda(K, x, prev, c0, c1, mask_a, mask_b, in_scale)
     out_dtype = out_dtype or x.dtype
+    acc = _acc(K.dtype)
+    c0 = None if c0 is None else c0.to(acc).contiguous()
+    c1 = None if c1 is None else c1.to(acc).contiguous()
+    in_scale = None if in_scale is None else in_scale.to(torch.float64).contiguous()
+    mask_a = None if mask_a is None else mask_a.to(torch.uint8).contiguous()
+    mask_b = None if mask_b is None else mask_b.to(torch.uint8).contiguous()
     n1, n2, n3 = _grid(x)
     y = torch.empty(x.shape, dtype=out_dtype, device=x.device)
     check(lib.mlpcg_sm_apply(ptr(K), code_of(K.dtype), ptr(x), code_of(x.dtype), ptr(y),
@@ -169,6 +175,9 @@ def avgpool2(x):
 
 def upsample2(x, mode="trilinear", scale=1.0, prev=None, c0=None, c1=None):
     require_cuda(x, prev, c0, c1)
+    acc = _acc(x.dtype)
+    c0 = None if c0 is None else c0.to(acc).contiguous()
+    c1 = None if c1 is None else c1.to(acc).contiguous()
     m1, m2, m3 = _grid(x)
     y = torch.empty(*x.shape[:-3], 2 * m1, 2 * m2, 2 * m3, dtype=x.dtype, device=x.device)
     check(lib.mlpcg_upsample2(ptr(x), ptr(y), _batch(x), 2 * m1, 2 * m2, 2 * m3,

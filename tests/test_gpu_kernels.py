@@ -50,17 +50,20 @@ def test_stencil_weights_transposed_f64(dev, C, shape):
     assert _rel(Kt, ref) < 1e-12
 
 
-@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-6), (torch.bfloat16, 2 ** -8),
-                                       (torch.float16, 2 ** -10)])
-def test_stencil_weights_storage_types(dev, dtype, tol):
+@pytest.mark.parametrize("dtype,rnd", [(torch.float32, 0.0), (torch.bfloat16, 2 ** -8),
+                                       (torch.float16, 2 ** -11)])
+def test_stencil_weights_storage_types(dev, dtype, rnd):
+    """FP32 accumulation (error <= 1e-6 of the term scale sum |W||img| + |b|), then one
+    rounding to the storage type (<= half an ulp of the value)."""
     from mlpcg_amd import kernels as K
     C, shape = 1, (16, 8, 32)
     img, W, b = _rand((C, *shape), 7).abs(), _rand((27, C, 3, 3, 3), 8), _rand((27,), 9)
     Kg = K.stencil_weights(img.float().to(dev), W.float().to(dev), b.float().to(dev), dtype=dtype)
     assert Kg.dtype == dtype
     Kr = M.stencil_weights(img.float().double(), W.float().double(), b.float().double())
-    err = (Kg.double().cpu() - Kr).abs() / Kr.abs().clamp(min=1e-3)
-    assert float(err.max()) < tol
+    scale = M.stencil_weights(img.float().double(), W.float().double().abs(), b.float().double().abs())
+    err = (Kg.double().cpu() - Kr).abs()
+    assert bool((err <= 1e-6 * scale + rnd * Kr.abs()).all())
 
 
 @pytest.mark.parametrize("trans", [False, True])
@@ -156,7 +159,7 @@ def test_avgpool2(dev, dtype, shape):
     x = _rand((3, 1, *shape), 31).to(dtype)
     y = K.avgpool2(x.to(dev))
     yr = F.avg_pool3d(x.double(), (2, 2, 2))
-    tol = {torch.float64: 1e-15, torch.float32: 1e-7, torch.bfloat16: 2 ** -8, torch.float16: 2 ** -11}[dtype]
+    tol = {torch.float64: 1e-15, torch.float32: 5e-7, torch.bfloat16: 2 ** -8, torch.float16: 2 ** -11}[dtype]
     assert _rel(y, yr) < tol
 
 

@@ -1,0 +1,304 @@
+#!/usr/bin/env python
+"""Benchmark: PCG time-to-1e-6 relative residual at 256^3 with the BF16 vs the FP32
+SM preconditioner, plus Adam train steps/s of the preconditioner.
+
+One step = one full solve of the 256^3 pillars scene (24 vertical solid pillars of
+radius 4-10, solid walls, air above z = 0.55 N) with the l=5 SmallSMModelDn3D
+preconditioner (C=1 image diag(A)/6): the image-dependent setup (pyramid, stencils,
+SMLinear coefficients) followed by npcg to ||b - Ax||/||b|| <= 1e-6, the residual
+recomputed in FP64 every iteration.  `value` is the mean seconds per step with the BF16
+preconditioner; inputs are resident in HBM before timing.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PCG time-to-1e-6 rel. residual at 256³, BF16 vs FP32 precond; train steps/s"
+HBM_PEAK_GBS = 8000.0
+DEFAULT_WEIGHTS = os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c1.pt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--method", default="npcg")
+    ap.add_argument("--max-it", type=int, default=300)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-train", action="store_true")
+    ap.add_argument("--skip-fp16", action="store_true")
+    ap.add_argument("--roofline-iters", type=int, default=20)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="run only the dominant kernel (for rocprofv3 counter passes)")
+    return ap.parse_args()
+
+
+def sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def timed(fn):
+    sync()
+    t0 = time.perf_counter()
+    out = fn()
+    sync()
+    return time.perf_counter() - t0, out
+
+
+def load_model(path, levels, dev):
+    import torch
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
+    source = "random-init"
+    if path and os.path.exists(path):
+        model.load_state_dict(torch.load(path, map_location=dev, weights_only=True))
+        source = os.path.relpath(path, ROOT)
+    else:
+        torch.manual_seed(0)
+        model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
+    return model.eval(), source
+
+
+def roofline_apply(model, image, r, dtype, iters):
+    """Level-0 SM block apply (first block of the preconditioner: FP32 input scaled by
+    1/||r||, K and output in `dtype`), timed with HIP events on torch's current stream,
+    which is the stream the library launches on."""
+    import torch
+    from mlpcg_amd import kernels as K
+    W, b = model.pre[0].weight, model.pre[0].bias
+    Kd = K.stencil_weights(image, W, b, dtype=dtype)
+    scale = torch.tensor(1.0 / float(r.norm()), dtype=torch.float64, device=r.device)
+    x = r.reshape(1, 1, *r.shape[-3:]).contiguous()
+    for _ in range(3):
+        K.sm_apply(Kd, x, out_dtype=dtype, in_scale=scale)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        K.sm_apply(Kd, x, out_dtype=dtype, in_scale=scale)
+    e1.record()
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / iters
+    V = x.numel()
+    esz = torch.tensor([], dtype=dtype).element_size()
+    alg = 27 * esz * V + 4 * V + esz * V
+    return t, alg
+
+
+def copy_bandwidth(dev, nbytes=2 << 30):
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / 10
+    return 2 * nbytes / t / 1e9
+
+
+def train_rate(dev, steps=3, warmup=1, n=128, scenes=8, levels=5):
+    """Adam steps/s on 8 seeded 128^3 scenes (one N(0,1) RHS each), loss
+    ||b - A P(b)||^2/||b||^2, hand-written backward kernels."""
+    import torch
+    from mlpcg_amd import train as T
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    torch.manual_seed(1)
+    model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
+    opt = torch.optim.Adam(model.parameters(), lr=1e-4)
+    batches = [T.SceneBatch(s, 1, dev) for s in T.make_scenes(n, range(200, 200 + scenes))]
+    gen = torch.Generator(device=dev).manual_seed(2)
+    rhs = [T.white_rhs(sb, 1, gen) for sb in batches]
+    for _ in range(warmup):
+        T.train_step(model, batches, rhs, opt)
+    t, _ = timed(lambda: [T.train_step(model, batches, rhs, opt) for _ in range(steps)])
+    return steps / t, torch.cuda.max_memory_allocated(dev)
+
+
+def cpu_baseline(flags, image_np, model, iters, levels):
+    """The oracle (FP32 torch on the host cores) for one 256^3 preconditioner apply plus
+    the per-iteration SpMVs and dots with scipy, extrapolated to `iters` iterations."""
+    import torch
+    import scipy.sparse  # noqa: F401
+    from mlpcg_amd import scenes as S
+    from oracle import model as M
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    P = {k: v.detach().float().cpu() for k, v in model.state_dict().items()}
+    img = torch.as_tensor(image_np, dtype=torch.float32)
+    A = S.build_matrix(flags)
+    fluid = flags.ravel() == S.FLUID
+    rng = np.random.default_rng(0)
+    r = rng.standard_normal(A.shape[0])
+    g = np.zeros(flags.size, dtype=np.float32)
+    g[fluid] = r / np.linalg.norm(r)
+    x = torch.as_tensor(g).reshape(1, 1, *flags.shape)
+    with torch.no_grad():
+        t_apply, _ = _wall(lambda: M.model_forward(P, levels, img, x))
+    t_spmv, _ = _wall(lambda: (A @ r, A @ r))
+    t_dots, _ = _wall(lambda: (r @ r, r @ r, r @ r))
+    per_it = t_apply + t_spmv + t_dots
+    return {"value": per_it * iters, "unit": "s", "cores": cores, "kind": "port",
+            "sample": (f"oracle FP32 torch preconditioner apply at 256^3 ({t_apply:.2f} s) + 2 scipy "
+                       f"SpMV + 3 dots ({t_spmv + t_dots:.3f} s) on {cores} host threads, "
+                       f"x {iters} iterations (extrapolated, FP32-preconditioner iteration count)"),
+            "per_iteration_s": per_it}
+
+
+def _wall(fn):
+    t0 = time.perf_counter()
+    out = fn()
+    return time.perf_counter() - t0, out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the preconditioned solve of one scene is a "
+              "single-GPU workload here (see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from mlpcg_amd import scenes as S
+    from mlpcg_amd.solver import FusedPCG
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    log(f"[bench] scene {args.n}^3 pillars seed {args.seed}")
+    sc = S.scene_pillars(args.n, args.seed)
+    model, wsrc = load_model(args.weights, args.levels, dev)
+    image_np = S.image_channels(sc.flags, 1)
+    image = torch.as_tensor(image_np, device=dev)
+    b_np = S.make_rhs(sc.flags, args.seed)[0]
+    solver = FusedPCG(sc.flags, "flags", device=dev)
+    b = solver.vector(b_np)
+    r0 = b.reshape(sc.shape)
+
+    if args.profile_only:
+        t, alg = roofline_apply(model, image, r0, torch.bfloat16, args.roofline_iters)
+        log(f"[bench] profile-only: sm_apply bf16 {t * 1e3:.3f} ms, {alg / t / 1e9:.1f} GB/s")
+        return
+
+    plans = {"bf16": model.make_preconditioner(image, torch.bfloat16),
+             "fp32": model.make_preconditioner(image, torch.float32)}
+    if not args.skip_fp16:
+        plans["fp16"] = model.make_preconditioner(image, torch.float16)
+
+    def step(name):
+        plans[name].update(image)
+        return solver.solve(b, plans[name], method=args.method, max_it=args.max_it, tol=args.tol)
+
+    results = {}
+    for name in plans:
+        for _ in range(args.warmup):
+            step(name)
+        k = args.steps if name in ("bf16", "fp32") else max(1, args.steps // 2)
+        times, infos = [], []
+        xs = None
+        for _ in range(k):
+            t, (xs, info) = timed(lambda: step(name))
+            times.append(t)
+            infos.append(info)
+        t_setup, _ = timed(lambda: plans[name].update(image))
+        results[name] = {"s_per_solve": float(np.mean(times)), "min_s": float(np.min(times)),
+                         "iters": infos[-1]["iters"], "status": infos[-1]["status"],
+                         "rel_res_device_fp64": infos[-1]["rel_res"], "setup_s": t_setup,
+                         "x": xs}
+        log(f"[bench] {name}: {results[name]['s_per_solve'] * 1e3:.2f} ms/solve, "
+            f"{results[name]['iters']} iterations, status {results[name]['status']}")
+
+    # FP64 check of the returned iterates on the host
+    A = S.build_matrix(sc.flags)
+    fluid = sc.fluid_cells
+    nb = np.linalg.norm(b_np)
+    for name, rres in results.items():
+        x = rres.pop("x").cpu().numpy()[fluid].astype(np.float64)
+        rres["rel_res_host_fp64"] = float(np.linalg.norm(b_np - A @ x) / nb)
+
+    # preconditioner error vs the FP64 plan on the same input
+    plan64 = model.make_preconditioner(image, torch.float64)
+    z64 = plan64.apply(r0.double() / r0.double().norm())
+    for name, p in plans.items():
+        z = p.apply(r0 / r0.norm()).double()
+        results[name]["precond_rel_err_vs_fp64"] = float((z - z64).norm() / z64.norm())
+    del plan64, z64
+
+    # roofline of the dominant kernel (level-0 SM apply, BF16)
+    t_k, alg = roofline_apply(model, image, r0, torch.bfloat16, args.roofline_iters)
+    achieved = alg / t_k / 1e9
+    copy_gbs = copy_bandwidth(dev)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "sm_apply_kernel<bf16,f32,bf16,bf16,VZ=8> level 0",
+            "avg_ms": round(t_k * 1e3, 4), "algorithmic_bytes": alg,
+            "measured_copy_GBs": round(copy_gbs, 1)}
+    log(f"[bench] roofline: {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of peak), copy {copy_gbs:.0f} GB/s")
+
+    train = None
+    if not args.skip_train:
+        sps, peak = train_rate(dev)
+        train = {"steps_per_s": round(sps, 3), "workload": "8 scenes x 128^3, 1 RHS each, l=5, FP32, Adam",
+                 "peak_hbm_GiB": round(peak / 2 ** 30, 2)}
+        log(f"[bench] train: {sps:.2f} steps/s")
+
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(sc.flags, image_np, model, results["fp32"]["iters"], args.levels)
+        log(f"[bench] cpu baseline: {cpu['value']:.1f} s (extrapolated)")
+
+    bf, f32 = results["bf16"], results["fp32"]
+    out = {
+        "metric": METRIC,
+        "value": round(bf["s_per_solve"], 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(bf["s_per_solve"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"{args.n}^3 pillars scene, l={args.levels} SM preconditioner "
+                               f"(BF16 storage, FP32 accumulate), {args.method} to {args.tol:g}",
+                   "grid": args.n, "levels": args.levels, "method": args.method,
+                   "weights": wsrc, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp32_s_per_solve": round(f32["s_per_solve"], 6),
+        "bf16_speedup_vs_fp32": round(f32["s_per_solve"] / bf["s_per_solve"], 3),
+        "iters": {k: v["iters"] for k, v in results.items()},
+        "iter_ratio_bf16_vs_fp32": round(bf["iters"] / max(f32["iters"], 1), 3),
+        "solves": results,
+        "train": train,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -161,6 +161,11 @@ int mlpcg_dot(const float* x, const float* y, long n, double* out, void* workspa
 int mlpcg_precond_create(mlpcg_precond** out, int kind, int n_levels, int C, int n1, int n2,
                          int n3, int dtype, int nearest, int mask, int swap, const void* image,
                          const void* const* weights, const void* const* biases, void* stream);
+/* Recompute everything that depends on the image or the weights (pyramid, stencils,
+ * c0/c1) for a new image of the same grid, reusing the plan's memory; same argument
+ * conventions as create. */
+int mlpcg_precond_update(mlpcg_precond* P, const void* image, const void* const* weights,
+                         const void* const* biases, void* stream);
 int mlpcg_precond_apply(mlpcg_precond* P, const void* r, void* z, const double* in_scale,
                         const uint8_t* out_mask, void* stream);
 size_t mlpcg_precond_bytes(const mlpcg_precond* P);

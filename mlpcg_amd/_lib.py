@@ -70,6 +70,7 @@ SIGNATURES = {
     "mlpcg_dot": (i32, [vp, vp, i64, vp, vp, vp]),
     "mlpcg_precond_create": (i32, [C.POINTER(vp), i32, i32, i32, i32, i32, i32, i32, i32, i32, i32,
                                    vp, C.POINTER(vp), C.POINTER(vp), vp]),
+    "mlpcg_precond_update": (i32, [vp, vp, C.POINTER(vp), C.POINTER(vp), vp]),
     "mlpcg_precond_apply": (i32, [vp, vp, vp, vp, vp, vp]),
     "mlpcg_precond_bytes": (sz, [vp]),
     "mlpcg_precond_destroy": (None, [vp]),

@@ -290,6 +290,12 @@ int mlpcg_precond_create(mlpcg_precond** out, int kind, int n_levels, int C, int
   return 0;
 }
 
+int mlpcg_precond_update(mlpcg_precond* P, const void* image, const void* const* weights,
+                         const void* const* biases, void* stream) {
+  MLPCG_REQUIRE(P && P->impl, "precond_update: null plan");
+  return P->impl->update(image, weights, biases, STREAM(stream));
+}
+
 int mlpcg_precond_apply(mlpcg_precond* P, const void* r, void* z, const double* in_scale,
                         const uint8_t* out_mask, void* stream) {
   MLPCG_REQUIRE(P && P->impl && r && z, "precond_apply: null pointer");

@@ -78,67 +78,82 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
     TRY(palloc(P, &P->tmp[l], (size_t)P->V[l] * es));
     if (kind == kSPD) TRY(palloc(P, &P->xs[l], (size_t)P->V[l] * es));
   }
-  // image pyramid in the accumulate precision
-  MLPCG_HIP(hipMemcpyAsync(P->img[0], image, (size_t)C * P->V[0] * as, hipMemcpyDeviceToDevice, st));
-  for (int l = 1; l <= n; ++l)
-    TRY(launch_avgpool2(P->acc, P->img[l - 1], P->img[l], C, P->dims[l - 1][0], P->dims[l - 1][1],
-                        P->dims[l - 1][2], nullptr, st));
   if (mask) {
     void* m = nullptr;
     TRY(palloc(P, &m, (size_t)P->V[0]));
     P->maskimg = (uint8_t*)m;
-    TRY(launch_nonzero_mask(P->acc, (const char*)P->img[0] + (size_t)P->V[0] * as, P->maskimg, P->V[0], st));
   }
   // stencils: (weight index, level, transposed)
-  struct Blk { int w, level, trans; };
-  std::vector<Blk> blocks;
-  int nw = 0;
   if (kind == kSmallSM) {
-    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 0});          // pre
-    for (int i = 0; i < n; ++i) blocks.push_back({n + i, i, 0});      // post
-    blocks.push_back({2 * n, n, 0});                                  // l
-    nw = 2 * n + 1;
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0});          // pre
+    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 0});      // post
+    P->blocks.push_back({2 * n, n, 0});                                  // l
+    P->nw = 2 * n + 1;
   } else if (kind == kSPD) {
-    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 0});          // l0
-    for (int i = 0; i < n; ++i) blocks.push_back({n + i, i, 0});      // l1
-    blocks.push_back({2 * n, n, 0});                                  // l
-    for (int i = 0; i < n; ++i) blocks.push_back({n + i, i, 1});      // l1_t
-    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 1});          // l0_t
-    blocks.push_back({2 * n, n, 1});                                  // l_t
-    nw = 2 * n + 1;
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0});          // l0
+    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 0});      // l1
+    P->blocks.push_back({2 * n, n, 0});                                  // l
+    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 1});      // l1_t
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 1});          // l0_t
+    P->blocks.push_back({2 * n, n, 1});                                  // l_t
+    P->nw = 2 * n + 1;
   } else {
-    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 0});          // l0
-    blocks.push_back({n, n, 0});                                      // l
-    for (int i = 0; i < n; ++i) blocks.push_back({i, i, 1});          // l0_t
-    blocks.push_back({n, n, 1});                                      // l_t
-    nw = n + 1;
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0});          // l0
+    P->blocks.push_back({n, n, 0});                                      // l
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 1});          // l0_t
+    P->blocks.push_back({n, n, 1});                                      // l_t
+    P->nw = n + 1;
   }
-  for (const Blk& b : blocks) {
+  for (const Precond::Blk& b : P->blocks) {
     void* K = nullptr;
     TRY(palloc(P, &K, (size_t)27 * P->V[b.level] * es));
     P->K.push_back(K);
-    const int* d = P->dims[b.level];
-    TRY(launch_stencil_weights2(P->acc, dtype, P->img[b.level], weights[b.w], biases[b.w], K, C, d[0],
-                                d[1], d[2], b.trans != 0, st));
+  }
+  TRY(palloc(P, &P->coef, (size_t)4 * n * as));
+  TRY(palloc(P, &P->lin_ws, mean_patch_workspace(P->acc, C, n1, n2, n3)));
+  TRY(palloc(P, &P->ybar, (size_t)27 * C * as));
+  TRY(P->update(image, weights, biases, st));
+#undef TRY
+  *out = P;
+  return 0;
+}
+
+// Everything that depends on the image (and the weights): pyramid, stencils, SMLinear
+// coefficients.  Reuses the buffers allocated at creation.
+int Precond::update(const void* image, const void* const* weights, const void* const* biases,
+                    hipStream_t st) {
+  MLPCG_REQUIRE(image && weights && biases, "precond_update: null pointer");
+  const size_t as = dsize(acc);
+  int rc;
+#define TRY(x)           \
+  do {                   \
+    rc = (x);            \
+    if (rc) return rc;   \
+  } while (0)
+  // image pyramid in the accumulate precision
+  MLPCG_HIP(hipMemcpyAsync(img[0], image, (size_t)C * V[0] * as, hipMemcpyDeviceToDevice, st));
+  for (int l = 1; l <= n; ++l)
+    TRY(launch_avgpool2(acc, img[l - 1], img[l], C, dims[l - 1][0], dims[l - 1][1], dims[l - 1][2],
+                        nullptr, st));
+  if (mask) TRY(launch_nonzero_mask(acc, (const char*)img[0] + (size_t)V[0] * as, maskimg, V[0], st));
+  for (size_t k = 0; k < blocks.size(); ++k) {
+    const Blk& b = blocks[k];
+    const int* d = dims[b.level];
+    TRY(launch_stencil_weights2(acc, dtype, img[b.level], weights[b.w], biases[b.w], K[k], C, d[0], d[1],
+                                d[2], b.trans != 0, st));
   }
   // SMLinear coefficients c0[i], c1[i] from img[i] (plus their squares for kSmallSPD)
-  TRY(palloc(P, &P->coef, (size_t)4 * n * as));
-  void* ws = nullptr;
-  TRY(palloc(P, &ws, mean_patch_workspace(P->acc, C, n1, n2, n3)));
-  void* ybar = nullptr;
-  TRY(palloc(P, &ybar, (size_t)27 * C * as));
   for (int which = 0; which < 2; ++which) {
     for (int i = 0; i < n; ++i) {
       const int widx = nw + which * n + i;
-      char* z = (char*)P->coef + (size_t)(which * n + i) * as;
-      const int* d = P->dims[i];
-      TRY(launch_sm_linear(P->acc, P->img[i], weights[widx], biases[widx], z, ybar, ws, C, d[0], d[1],
-                           d[2], st));
+      char* z = (char*)coef + (size_t)(which * n + i) * as;
+      const int* d = dims[i];
+      TRY(launch_sm_linear(acc, img[i], weights[widx], biases[widx], z, ybar, lin_ws, C, d[0], d[1], d[2],
+                           st));
     }
   }
-  if (kind == kSmallSPD) TRY(launch_square(P->acc, P->coef, (char*)P->coef + (size_t)2 * n * as, 2 * n, st));
+  if (kind == kSmallSPD) TRY(launch_square(acc, coef, (char*)coef + (size_t)2 * n * as, 2 * n, st));
 #undef TRY
-  *out = P;
   return 0;
 }
 

@@ -31,9 +31,18 @@ struct Precond {
   void* tmp[kMaxLevels + 1] = {};   // transfer scratch (storage type)
   void* xs[kMaxLevels + 1] = {};    // SPD second-half vectors (storage type)
   uint8_t* maskimg = nullptr;       // level-0 model mask: img[1] != 0
+  struct Blk {
+    int w, level, trans;            // weight index, level, transposed stencil
+  };
+  std::vector<Blk> blocks;          // one per entry of K
+  int nw = 0;                       // number of block weights before c0/c1
+  void* lin_ws = nullptr;           // SMLinear reduction workspace
+  void* ybar = nullptr;             // SMLinear mean patch scratch
   size_t bytes = 0;
   std::vector<void*> allocs;
 
+  int update(const void* image, const void* const* weights, const void* const* biases,
+             hipStream_t st);
   int apply(const float* r, float* z, const double* in_scale, const uint8_t* out_mask,
             const int* done, hipStream_t st);
   ~Precond();

@@ -57,6 +57,22 @@ class Preconditioner:
                                        warr, barr, stream_ptr(self.device)), "precond_create")
         self._h = handle
 
+    def update(self, image, model=None):
+        """Recompute pyramid, stencils and coefficients for a new image (same grid) and,
+        if `model` is given, its current parameters; no allocation in the library."""
+        if model is not None:
+            ws, bs = model.plan_params()
+            for dst, src in zip(self._w, ws):
+                dst.copy_(src.detach())
+            for dst, src in zip(self._b, bs):
+                dst.copy_(src.detach())
+        self._img.copy_(image.detach())
+        warr = (C.c_void_p * len(self._w))(*[ptr(w) for w in self._w])
+        barr = (C.c_void_p * len(self._b))(*[ptr(b) for b in self._b])
+        check(lib.mlpcg_precond_update(self._h, ptr(self._img), warr, barr, stream_ptr(self.device)),
+              "precond_update")
+        return self
+
     @property
     def handle(self):
         return self._h

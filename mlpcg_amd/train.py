@@ -1,0 +1,228 @@
+"""Training of the SM preconditioner on synthetic scenes.
+
+Loss of the benchmark (configured form): mean over the batch of ||b - A P(b)||^2 / ||b||^2
+with A the fluid Laplacian applied matrix-free; the reference's residual_loss
+(loss_functions.py:18-23, mean of ||b - A P(b)||) is available as well.  The step is
+forward -> loss -> backward (hand-written HIP backward kernels behind the autograd
+Functions of sm_model_3d, stencils recomputed in backward) -> Adam, as train.py:61-72.
+
+    python -m mlpcg_amd.train --out mlpcg_amd/weights/sm_l5_c1.pt --steps 3000
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import kernels as K
+from . import scenes as S
+from .sm_model_3d import SmallSMModelDn3D, SmallSPDSMModelDn3D, SPDSMModelDn3D
+
+MODELS = {"sm": lambda n, c: SmallSMModelDn3D(n, c, "trilinear"),
+          "smallspd": lambda n, c: SmallSPDSMModelDn3D(n, c),
+          "spd": lambda n, c: SPDSMModelDn3D(n, c)}
+
+
+class LaplacianFunction(torch.autograd.Function):
+    """y = A x on the full grid (A symmetric, so the backward is A g).  x: (B, V) float32."""
+
+    @staticmethod
+    def forward(ctx, x, code):
+        ctx.save_for_backward(code)
+        return torch.stack([K.spmv_flags(code, xi.contiguous()) for xi in x])
+
+    @staticmethod
+    def backward(ctx, g):
+        code, = ctx.saved_tensors
+        return torch.stack([K.spmv_flags(code, gi.contiguous()) for gi in g]), None
+
+
+class SceneBatch:
+    """Device-resident scene data: flags code, fluid mask, image channels."""
+
+    def __init__(self, scene: S.Scene, num_imgs: int, device, image_dtype=torch.float32):
+        self.scene = scene
+        self.shape = scene.shape
+        self.flags = torch.as_tensor(scene.flags.astype(np.int8), device=device)
+        self.code = K.flags_code(self.flags)
+        self.fluid = (self.flags == S.FLUID)
+        self.image = torch.as_tensor(S.image_channels(scene.flags, num_imgs), device=device).to(image_dtype)
+        lab, closed = S.fluid_components_without_air(scene.flags)
+        self.closed = [torch.as_tensor((lab == c).ravel(), device=device) for c in closed]
+
+
+def white_rhs(sb: SceneBatch, count: int, gen: torch.Generator):
+    """N(0,1) on fluid cells, mean-subtracted on air-free components, unit norm."""
+    V = sb.fluid.numel()
+    b = torch.randn(count, V, generator=gen, device=sb.fluid.device) * sb.fluid.reshape(1, V)
+    return _finish_rhs(sb, b)
+
+
+def multiscale_rhs(sb: SceneBatch, count: int, gen: torch.Generator, levels: int = 5):
+    """Sum of white noise drawn on the grid and on 2^l-coarser grids, upsampled
+    (trilinear); gives every wavelength a comparable share of the energy, which is what
+    the reference's low-mode-weighted Ritz right-hand sides provide (preprocess.py:38-58)."""
+    n1, n2, n3 = sb.shape
+    dev = sb.fluid.device
+    b = torch.zeros(count, 1, n1, n2, n3, device=dev)
+    for l in range(levels + 1):
+        f = 2 ** l
+        if n1 % f or n2 % f or n3 % f:
+            break
+        z = torch.randn(count, 1, n1 // f, n2 // f, n3 // f, generator=gen, device=dev)
+        for _ in range(l):
+            z = K.upsample2(z.contiguous(), "trilinear")
+        b += z
+    V = n1 * n2 * n3
+    b = b.reshape(count, V) * sb.fluid.reshape(1, V)
+    return _finish_rhs(sb, b)
+
+
+def _finish_rhs(sb, b):
+    for m in sb.closed:
+        mm = m.reshape(1, -1)
+        mean = (b * mm).sum(1, keepdim=True) / mm.sum()
+        b = b - mean * mm
+    return b / b.norm(dim=1, keepdim=True).clamp(min=1e-30)
+
+
+def relative_residual_sq(sb: SceneBatch, x, b):
+    """mean_b ||b - A x||^2 / ||b||^2 with x restricted to the fluid cells."""
+    B = b.shape[0]
+    V = b.shape[1]
+    xf = x.reshape(B, V).float() * sb.fluid.reshape(1, V)
+    r = b - LaplacianFunction.apply(xf, sb.code)
+    return ((r * r).sum(1) / (b * b).sum(1)).mean()
+
+
+def residual_loss(sb: SceneBatch, x, b):
+    """loss_functions.py:18-23: mean_b ||b - A x||."""
+    B, V = b.shape
+    xf = x.reshape(B, V).float() * sb.fluid.reshape(1, V)
+    r = b - LaplacianFunction.apply(xf, sb.code)
+    return r.norm(dim=1).mean()
+
+
+def train_step(model, batches, rhs, optimizer, act_dtype=torch.float32, loss_fn=relative_residual_sq):
+    """One Adam step over a list of (SceneBatch, rhs[B, V]) pairs; returns the mean loss."""
+    optimizer.zero_grad(set_to_none=True)
+    total = 0.0
+    for sb, b in zip(batches, rhs):
+        x = model(sb.image.to(act_dtype), b.reshape(b.shape[0], 1, *sb.shape).to(act_dtype))
+        loss = loss_fn(sb, x, b) / len(batches)
+        loss.backward()
+        total += float(loss.detach())
+    optimizer.step()
+    return total
+
+
+def make_scenes(n, seeds, kind="spheres"):
+    out = []
+    for s in seeds:
+        if kind == "spheres":
+            out.append(S.scene_spheres(n, s))
+        elif kind == "pillars":
+            out.append(S.scene_pillars(n, s))
+        elif kind == "spheres40":
+            out.append(S.scene_for_config(4, s, n))
+        else:
+            raise ValueError(kind)
+    return out
+
+
+def evaluate_pcg(model, sb: SceneBatch, dtype=torch.float32, method="npcg", max_it=300, tol=1e-6, seed=0):
+    from .solver import FusedPCG
+    plan = model.make_preconditioner(sb.image.float(), dtype)
+    solver = FusedPCG(sb.scene.flags, "flags", device=sb.flags.device)
+    b = S.make_rhs(sb.scene.flags, seed)[0]
+    x, info = solver.solve(solver.vector(b), plan, method=method, max_it=max_it, tol=tol)
+    return info
+
+
+def evaluate_methods(model, sb: SceneBatch, methods=("npcg", "npcg_flex", "npsd"), **kw):
+    out = {}
+    for m in methods:
+        info = evaluate_pcg(model, sb, method=m, **kw)
+        out[m] = (info["iters"], info["rel_res"])
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--num-imgs", type=int, default=1)
+    ap.add_argument("--scenes", type=int, default=16)
+    ap.add_argument("--rhs-per-scene", type=int, default=2)
+    ap.add_argument("--scenes-per-step", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--eval-every", type=int, default=250)
+    ap.add_argument("--max-minutes", type=float, default=15.0)
+    ap.add_argument("--init", default=None)
+    ap.add_argument("--loss", choices=("relsq", "resid"), default="relsq")
+    ap.add_argument("--rhs", choices=("mix", "white", "multiscale"), default="mix")
+    ap.add_argument("--select", default="npcg", help="method whose iteration count selects the kept weights")
+    ap.add_argument("--model", choices=tuple(MODELS), default="sm")
+    args = ap.parse_args(argv)
+    loss_fn = relative_residual_sq if args.loss == "relsq" else residual_loss
+
+    dev = torch.device("cuda:0")
+    torch.manual_seed(args.seed)
+    model = MODELS[args.model](args.levels, args.num_imgs).to(dev)
+    if args.init:
+        model.load_state_dict(torch.load(args.init, map_location=dev, weights_only=True))
+    opt = torch.optim.Adam(model.parameters(), lr=args.lr)
+    sched = torch.optim.lr_scheduler.LambdaLR(
+        opt, lambda s: 0.5 * (1 + math.cos(math.pi * min(s, args.steps) / args.steps)) * 0.99 + 0.01)
+    scenes = [SceneBatch(s, args.num_imgs, dev) for s in make_scenes(args.n, range(100, 100 + args.scenes))]
+    held = SceneBatch(S.scene_spheres(args.n, 7), args.num_imgs, dev)
+    gen = torch.Generator(device=dev).manual_seed(args.seed)
+    t0 = time.time()
+    log = []
+    best = None
+    for step in range(1, args.steps + 1):
+        idx = [(step * args.scenes_per_step + k) % len(scenes) for k in range(args.scenes_per_step)]
+        bs = [scenes[i] for i in idx]
+        rhs = []
+        for sb in bs:
+            h = {"mix": args.rhs_per_scene // 2, "white": 0, "multiscale": args.rhs_per_scene}[args.rhs]
+            parts = []
+            if args.rhs_per_scene - h:
+                parts.append(white_rhs(sb, args.rhs_per_scene - h, gen))
+            if h:
+                parts.append(multiscale_rhs(sb, h, gen))
+            rhs.append(torch.cat(parts))
+        loss = train_step(model, bs, rhs, opt, loss_fn=loss_fn)
+        sched.step()
+        if step % 25 == 0:
+            print(f"step {step} loss {loss:.5f} lr {sched.get_last_lr()[0]:.2e} t {time.time() - t0:.1f}s",
+                  flush=True)
+        if step % args.eval_every == 0 or step == args.steps:
+            with torch.no_grad():
+                ev = evaluate_methods(model, held)
+            rec = {"step": step, "loss": loss, "eval": ev, "time_s": time.time() - t0}
+            log.append(rec)
+            print(json.dumps(rec), flush=True)
+            it, res = ev[args.select]
+            if res <= 1e-6 and (best is None or it < best[0]):
+                best = (it, {k: v.detach().cpu().clone() for k, v in model.state_dict().items()})
+            os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+            sd = best[1] if best is not None else {k: v.detach().cpu() for k, v in model.state_dict().items()}
+            torch.save(sd, args.out)
+        if (time.time() - t0) / 60 > args.max_minutes:
+            print("time limit reached", flush=True)
+            break
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+        json.dump({"args": vars(args), "log": log, "best_iters": best[0] if best else None}, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--roofline-iters", type=int, default=20)
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the dominant kernel (for rocprofv3 counter passes)")
+    ap.add_argument("--workload", default="pillars256",
+                    choices=("pillars256", "spheres128", "train256", "sphere64cpu"),
+                    help="pillars256: the benchmark line; the others print their own report")
+    ap.add_argument("--train-steps", type=int, default=300)
     return ap.parse_args()
 
 
@@ -176,6 +180,104 @@ def _wall(fn):
     return time.perf_counter() - t0, out
 
 
+def workload_spheres128(args, dev):
+    """128^3, walls + 6 seeded spheres (r 6-20), seeded water level; l=5 FP32
+    preconditioner; the CSR SpMV and the matrix-free SpMV solves side by side."""
+    import torch
+    from mlpcg_amd import scenes as S
+    from mlpcg_amd.solver import FusedPCG
+    sc = S.scene_spheres(128, args.seed)
+    model, wsrc = load_model(args.weights, args.levels, dev)
+    image = torch.as_tensor(S.image_channels(sc.flags, 1), device=dev)
+    A = S.build_matrix(sc.flags)
+    b_np = S.make_rhs(sc.flags, args.seed)[0]
+    out = {"workload": "128^3 six spheres, l=5 FP32 preconditioner", "weights": wsrc, "method": args.method}
+    plan = model.make_preconditioner(image, torch.float32)
+    for layout in ("flags", "csr"):
+        s = FusedPCG(sc.flags, layout, A=A if layout == "csr" else None, device=dev)
+        b = s.vector(b_np)
+        for precond, tag in ((plan, "mlpcg"), (None, "cg")):
+            s.solve(b, precond, method=args.method, max_it=2000, tol=args.tol)
+            ts, infos = [], []
+            for _ in range(args.steps):
+                t, (x, info) = timed(lambda: s.solve(b, precond, method=args.method, max_it=2000, tol=args.tol))
+                ts.append(t)
+                infos.append(info)
+            xs = x.cpu().numpy().astype(np.float64)
+            if layout == "flags":
+                xs = xs[sc.fluid_cells]
+            res = float(np.linalg.norm(b_np - A @ xs) / np.linalg.norm(b_np))
+            out[f"{layout}_{tag}"] = {"s_per_solve": float(np.mean(ts)), "iters": infos[-1]["iters"],
+                                      "rel_res_host_fp64": res}
+            log(f"[bench] spheres128 {layout} {tag}: {np.mean(ts) * 1e3:.2f} ms, {infos[-1]['iters']} it, res {res:.2e}")
+    print(json.dumps(out), flush=True)
+
+
+def workload_train256(args, dev):
+    """Training at 256^3 (4 scenes, 40 spheres r 4-30, air fraction 20-50 %): BF16
+    forward/backward with FP32 master weights vs FP32, stencils recomputed in backward;
+    steps/s, peak HBM and PCG iterations on 8 held-out 256^3 scenes after the same
+    number of steps from the same starting weights."""
+    import torch
+    from mlpcg_amd import train as T
+    from mlpcg_amd.solver import FusedPCG
+    from mlpcg_amd import scenes as S
+    scenes = [T.SceneBatch(s, 1, dev) for s in T.make_scenes(256, range(300, 304), "spheres40")]
+    held = [T.SceneBatch(s, 1, dev) for s in T.make_scenes(256, range(400, 408), "spheres40")]
+    out = {"workload": "train 4 x 256^3 (40 spheres), l=5", "steps": args.train_steps}
+    gen = torch.Generator(device=dev).manual_seed(3)
+    for act_name, act in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
+        model, wsrc = load_model(args.weights, args.levels, dev)
+        model.train()
+        opt = torch.optim.Adam(model.parameters(), lr=3e-4)
+        torch.cuda.reset_peak_memory_stats(dev)
+        rhs = [T.white_rhs(sb, 1, gen) for sb in scenes]
+        T.train_step(model, scenes, rhs, opt, act_dtype=act)
+        t0 = time.perf_counter()
+        for k in range(args.train_steps):
+            rhs = [T.white_rhs(sb, 1, gen) for sb in scenes]
+            loss = T.train_step(model, scenes, rhs, opt, act_dtype=act)
+            if k % 25 == 0:
+                log(f"[bench] train256 {act_name} step {k} loss {loss:.4f}")
+        sync()
+        sps = args.train_steps / (time.perf_counter() - t0)
+        peak = torch.cuda.max_memory_allocated(dev) / 2 ** 30
+        its = []
+        model.eval()
+        with torch.no_grad():
+            for sb in held:
+                plan = model.make_preconditioner(sb.image, torch.float32)
+                s = FusedPCG(sb.scene.flags, "flags", device=dev)
+                _, info = s.solve(s.vector(S.make_rhs(sb.scene.flags, 0)[0]), plan, method=args.method,
+                                  max_it=args.max_it, tol=args.tol)
+                its.append(info["iters"] if info["status"] == 0 else None)
+        out[act_name] = {"steps_per_s": round(sps, 3), "peak_hbm_GiB": round(peak, 2), "last_loss": loss,
+                         "heldout_iters": its, "start_weights": wsrc}
+        log(f"[bench] train256 {act_name}: {sps:.2f} steps/s, peak {peak:.1f} GiB, held-out iters {its}")
+    print(json.dumps(out), flush=True)
+
+
+def workload_sphere64cpu(args):
+    """64^3 sphere scene, l=4, seeded random FP64 weights, C=1: the oracle's npcg on the
+    host CPU in FP64 (<= 200 iterations or 1e-6)."""
+    import torch
+    from mlpcg_amd import scenes as S
+    from oracle import model as M
+    from oracle import solvers as O
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    sc = S.scene_sphere64()
+    A = S.build_matrix(sc.flags)
+    b = S.make_rhs(sc.flags, 0)[0]
+    P = M.init_params(4, 1, torch.Generator().manual_seed(0))
+    img = torch.as_tensor(S.image_channels(sc.flags, 1)).double()
+    predict = O.make_predict(lambda x: M.model_forward(P, 4, img, x), sc.fluid_cells, sc.shape)
+    t, (x, k, hist) = _wall(lambda: O.npcg(b, A, np.zeros_like(b), predict, 200, tol=1e-6))
+    out = {"workload": "64^3 sphere, l=4 random-init FP64, CPU oracle npcg", "seconds": t, "iters": k,
+           "final_rel_res": float(np.linalg.norm(b - A @ x) / np.linalg.norm(b)), "cores": cores}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -186,8 +288,17 @@ def main():
     from mlpcg_amd import scenes as S
     from mlpcg_amd.solver import FusedPCG
 
+    if args.workload == "sphere64cpu":
+        workload_sphere64cpu(args)
+        return
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    if args.workload == "spheres128":
+        workload_spheres128(args, dev)
+        return
+    if args.workload == "train256":
+        workload_train256(args, dev)
+        return
     log(f"[bench] scene {args.n}^3 pillars seed {args.seed}")
     sc = S.scene_pillars(args.n, args.seed)
     model, wsrc = load_model(args.weights, args.levels, dev)

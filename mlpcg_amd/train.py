@@ -171,6 +171,9 @@ def main(argv=None):
     ap.add_argument("--rhs", choices=("mix", "white", "multiscale"), default="mix")
     ap.add_argument("--select", default="npcg", help="method whose iteration count selects the kept weights")
     ap.add_argument("--model", choices=tuple(MODELS), default="sm")
+    ap.add_argument("--act-dtype", choices=("fp32", "bf16"), default="fp32",
+                    help="activation/stencil storage in forward and backward (weights stay FP32)")
+    ap.add_argument("--kind", choices=("spheres", "pillars", "spheres40"), default="spheres")
     args = ap.parse_args(argv)
     loss_fn = relative_residual_sq if args.loss == "relsq" else residual_loss
 
@@ -182,8 +185,10 @@ def main(argv=None):
     opt = torch.optim.Adam(model.parameters(), lr=args.lr)
     sched = torch.optim.lr_scheduler.LambdaLR(
         opt, lambda s: 0.5 * (1 + math.cos(math.pi * min(s, args.steps) / args.steps)) * 0.99 + 0.01)
-    scenes = [SceneBatch(s, args.num_imgs, dev) for s in make_scenes(args.n, range(100, 100 + args.scenes))]
-    held = SceneBatch(S.scene_spheres(args.n, 7), args.num_imgs, dev)
+    act = torch.bfloat16 if args.act_dtype == "bf16" else torch.float32
+    scenes = [SceneBatch(s, args.num_imgs, dev)
+              for s in make_scenes(args.n, range(100, 100 + args.scenes), args.kind)]
+    held = SceneBatch(make_scenes(args.n, [7], args.kind)[0], args.num_imgs, dev)
     gen = torch.Generator(device=dev).manual_seed(args.seed)
     t0 = time.time()
     log = []
@@ -200,7 +205,7 @@ def main(argv=None):
             if h:
                 parts.append(multiscale_rhs(sb, h, gen))
             rhs.append(torch.cat(parts))
-        loss = train_step(model, bs, rhs, opt, loss_fn=loss_fn)
+        loss = train_step(model, bs, rhs, opt, act_dtype=act, loss_fn=loss_fn)
         sched.step()
         if step % 25 == 0:
             print(f"step {step} loss {loss:.5f} lr {sched.get_last_lr()[0]:.2e} t {time.time() - t0:.1f}s",

@@ -55,6 +55,7 @@ def parse():
                     choices=("pillars256", "spheres128", "train256", "sphere64cpu"),
                     help="pillars256: the benchmark line; the others print their own report")
     ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--graphs", type=int, default=0, help="replay solver iterations as HIP graphs")
     return ap.parse_args()
 
 
@@ -194,7 +195,8 @@ def workload_spheres128(args, dev):
     out = {"workload": "128^3 six spheres, l=5 FP32 preconditioner", "weights": wsrc, "method": args.method}
     plan = model.make_preconditioner(image, torch.float32)
     for layout in ("flags", "csr"):
-        s = FusedPCG(sc.flags, layout, A=A if layout == "csr" else None, device=dev)
+        s = FusedPCG(sc.flags, layout, A=A if layout == "csr" else None, device=dev,
+                     graphs=bool(args.graphs))
         b = s.vector(b_np)
         for precond, tag in ((plan, "mlpcg"), (None, "cg")):
             s.solve(b, precond, method=args.method, max_it=2000, tol=args.tol)
@@ -305,7 +307,7 @@ def main():
     image_np = S.image_channels(sc.flags, 1)
     image = torch.as_tensor(image_np, device=dev)
     b_np = S.make_rhs(sc.flags, args.seed)[0]
-    solver = FusedPCG(sc.flags, "flags", device=dev)
+    solver = FusedPCG(sc.flags, "flags", device=dev, graphs=bool(args.graphs))
     b = solver.vector(b_np)
     r0 = b.reshape(sc.shape)
 
@@ -407,7 +409,7 @@ def main():
         "config": {"workload": f"{args.n}^3 pillars scene, l={args.levels} SM preconditioner "
                                f"(BF16 storage, FP32 accumulate), {args.method} to {args.tol:g}",
                    "grid": args.n, "levels": args.levels, "method": args.method,
-                   "weights": wsrc, "parallelism": "single GPU"},
+                   "weights": wsrc, "graphs": bool(args.graphs), "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp32_s_per_solve": round(f32["s_per_solve"], 6),

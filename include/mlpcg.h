@@ -187,6 +187,10 @@ int mlpcg_solver_create_csr(mlpcg_solver** out, const int* rowptr, const int* co
 int mlpcg_solver_solve(mlpcg_solver* S, mlpcg_precond* P, const float* b, const float* x0,
                        float* x, int method, int max_it, double tol, double atol,
                        double* history, mlpcg_solve_info* info, void* stream);
+/* No reference counterpart: replay each iteration as a captured HIP graph (one per
+ * buffer parity, re-captured when P, b or the method change) on the solver's own
+ * stream, ordered after `stream`.  Same results as direct launches. */
+int mlpcg_solver_set_graphs(mlpcg_solver* S, int enable);
 void mlpcg_solver_destroy(mlpcg_solver* S);
 
 #ifdef __cplusplus

@@ -78,6 +78,7 @@ SIGNATURES = {
     "mlpcg_solver_create_csr": (i32, [C.POINTER(vp), vp, vp, vp, i64, vp, i32, i32, i32, vp]),
     "mlpcg_solver_solve": (i32, [vp, vp, vp, vp, vp, i32, i32, f64, f64, vp, C.POINTER(SolveInfo),
                                  vp]),
+    "mlpcg_solver_set_graphs": (i32, [vp, i32]),
     "mlpcg_solver_destroy": (None, [vp]),
 }
 

@@ -347,6 +347,13 @@ int mlpcg_solver_solve(mlpcg_solver* S, mlpcg_precond* P, const float* b, const 
   return 0;
 }
 
+int mlpcg_solver_set_graphs(mlpcg_solver* S, int enable) {
+  MLPCG_REQUIRE(S && S->impl, "solver_set_graphs: null solver");
+  S->impl->use_graphs = enable ? 1 : 0;
+  if (!enable) S->impl->drop_graphs();
+  return 0;
+}
+
 void mlpcg_solver_destroy(mlpcg_solver* S) {
   if (!S) return;
   delete S->impl;

@@ -86,9 +86,18 @@ struct Solver {
   hipStream_t side = nullptr;
   hipEvent_t ev_iter[4] = {}, ev_stat[4] = {};
   std::vector<void*> allocs;
+  // HIP-graph replay of the iteration (two graphs: even / odd buffer parity)
+  int use_graphs = 0;
+  hipStream_t work = nullptr;
+  hipEvent_t ev_in = nullptr;
+  hipGraphExec_t graph_exec[2] = {};
+  const Precond* graph_P = nullptr;
+  const float* graph_b = nullptr;
+  int graph_method = -1;
 
   int solve(Precond* P, const float* b, const float* x0, float* x_out, int method, int max_it,
             double tol, double atol, double* history_out, SolveInfo* info, hipStream_t st);
+  void drop_graphs();
   ~Solver();
 };
 

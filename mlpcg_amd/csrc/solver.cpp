@@ -18,6 +18,10 @@ namespace mlpcg {
 constexpr int kLookahead = 2;
 
 Solver::~Solver() {
+  drop_graphs();
+  if (work) (void)hipStreamSynchronize(work);
+  if (work) (void)hipStreamDestroy(work);
+  if (ev_in) (void)hipEventDestroy(ev_in);
   if (side) (void)hipStreamSynchronize(side);
   for (int i = 0; i < 4; ++i) {
     if (ev_iter[i]) (void)hipEventDestroy(ev_iter[i]);
@@ -53,6 +57,8 @@ static int solver_common(Solver* S, hipStream_t st) {
   MLPCG_HIP(hipHostMalloc((void**)&S->st_host, 4 * sizeof(SolveState), hipHostMallocDefault));
   memset(S->st_host, 0, 4 * sizeof(SolveState));
   MLPCG_HIP(hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking));
+  MLPCG_HIP(hipStreamCreateWithFlags(&S->work, hipStreamNonBlocking));
+  MLPCG_HIP(hipEventCreateWithFlags(&S->ev_in, hipEventDisableTiming));
   for (int i = 0; i < 4; ++i) {
     MLPCG_HIP(hipEventCreateWithFlags(&S->ev_iter[i], hipEventDisableTiming));
     MLPCG_HIP(hipEventCreateWithFlags(&S->ev_stat[i], hipEventDisableTiming));
@@ -118,6 +124,15 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
     history = (double*)h;
     hist_cap = max_it + 1;
   }
+  // Graph replay runs on the solver's own stream (the legacy default stream cannot be
+  // captured); it is ordered after the caller's stream and the solve synchronises it.
+  hipStream_t run = st;
+  if (use_graphs) {
+    MLPCG_HIP(hipEventRecord(ev_in, st));
+    MLPCG_HIP(hipStreamWaitEvent(work, ev_in, 0));
+    run = work;
+    if (graph_P != P || graph_b != b || graph_method != method) drop_graphs();
+  }
   // initial state
   SolveState init;
   memset(&init, 0, sizeof(init));
@@ -129,34 +144,34 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
   SolveState* hs = st_host;  // slot 0 carries the initial state up
   MLPCG_HIP(hipStreamSynchronize(side));
   *hs = init;
-  MLPCG_HIP(hipMemcpyAsync(this->st, hs, sizeof(SolveState), hipMemcpyHostToDevice, st));
+  MLPCG_HIP(hipMemcpyAsync(this->st, hs, sizeof(SolveState), hipMemcpyHostToDevice, run));
   const size_t vb = (size_t)n * sizeof(float);
-  if (x0) MLPCG_HIP(hipMemcpyAsync(x[0], x0, vb, hipMemcpyDeviceToDevice, st));
-  else MLPCG_HIP(hipMemsetAsync(x[0], 0, vb, st));
-  MLPCG_HIP(hipMemsetAsync(p[0], 0, vb, st));
-  MLPCG_HIP(hipMemsetAsync(p[1], 0, vb, st));
+  if (x0) MLPCG_HIP(hipMemcpyAsync(x[0], x0, vb, hipMemcpyDeviceToDevice, run));
+  else MLPCG_HIP(hipMemsetAsync(x[0], 0, vb, run));
+  MLPCG_HIP(hipMemsetAsync(p[0], 0, vb, run));
+  MLPCG_HIP(hipMemsetAsync(p[1], 0, vb, run));
   const int* done = &this->st->done;
 
-  FlatArgs fa;
-  memset(&fa, 0, sizeof(fa));
-  fa.code = code;
-  fa.b = b;
-  fa.st = this->st;
-  fa.partials = partials;
-  fa.counter = counter;
-  fa.n1 = n1;
-  fa.n2 = n2;
-  fa.n3 = n3;
-  CsrArgs ca;
-  memset(&ca, 0, sizeof(ca));
-  ca.rowptr = rowptr;
-  ca.col = col;
-  ca.val = val;
-  ca.b = b;
-  ca.st = this->st;
-  ca.partials = partials;
-  ca.counter = counter;
-  ca.n = n;
+  FlatArgs fa0;
+  memset(&fa0, 0, sizeof(fa0));
+  fa0.code = code;
+  fa0.b = b;
+  fa0.st = this->st;
+  fa0.partials = partials;
+  fa0.counter = counter;
+  fa0.n1 = n1;
+  fa0.n2 = n2;
+  fa0.n3 = n3;
+  CsrArgs ca0;
+  memset(&ca0, 0, sizeof(ca0));
+  ca0.rowptr = rowptr;
+  ca0.col = col;
+  ca0.val = val;
+  ca0.b = b;
+  ca0.st = this->st;
+  ca0.partials = partials;
+  ca0.counter = counter;
+  ca0.n = n;
 
   int rc = 0;
 #define TRY(x)         \
@@ -166,99 +181,141 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
   } while (0)
   // r_0 = b - A x_0
   if (layout == 0) {
+    FlatArgs fa = fa0;
     fa.x_old = x[0];
     fa.r = r[0];
-    TRY(launch_resid_flat(fa, 1, st));
+    TRY(launch_resid_flat(fa, 1, run));
   } else {
+    CsrArgs ca = ca0;
     ca.x_old = x[0];
     ca.r = r[0];
-    TRY(launch_resid_csr(ca, 1, st));
+    TRY(launch_resid_csr(ca, 1, run));
   }
 
-  auto precondition = [&](const float* rin, float* zout) -> int {
+  auto precondition = [&](const float* rin, float* zout, hipStream_t s) -> int {
     if (!P) {
-      MLPCG_HIP(hipMemcpyAsync(zout, rin, vb, hipMemcpyDeviceToDevice, st));
+      MLPCG_HIP(hipMemcpyAsync(zout, rin, vb, hipMemcpyDeviceToDevice, s));
       return 0;
     }
     const double* scale = &this->st->scale;
-    if (layout == 0) return P->apply(rin, zout, scale, code, done, st);
-    int rc2 = launch_scatter(fluid_idx, rin, grid_in, n, done, st);
+    if (layout == 0) return P->apply(rin, zout, scale, code, done, s);
+    int rc2 = launch_scatter(fluid_idx, rin, grid_in, n, done, s);
     if (rc2) return rc2;
-    rc2 = P->apply(grid_in, grid_out, scale, nullptr, done, st);
+    rc2 = P->apply(grid_in, grid_out, scale, nullptr, done, s);
     if (rc2) return rc2;
-    return launch_gather(fluid_idx, grid_out, zout, n, done, st);
+    return launch_gather(fluid_idx, grid_out, zout, n, done, s);
   };
 
-  int k = 0;
-  int npsd_count = 0;  // directions kept (npsd)
-  for (k = 1; k <= max_it; ++k) {
+  // One iteration (k counts from 1; cnt = directions created so far, npsd only).
+  auto enqueue_iter = [&](int k, int cnt, hipStream_t s) -> int {
     const int cur = (k - 1) & 1, nxt = k & 1;
     const float* rk = r[cur];
+    FlatArgs fa = fa0;
+    CsrArgs ca = ca0;
+    int rc2 = 0;
+#define TRY2(x)          \
+  do {                   \
+    rc2 = (x);           \
+    if (rc2) return rc2; \
+  } while (0)
     if (method != kNpsd) {
-      TRY(precondition(rk, z));
-      TRY(launch_rz_dot(rk, z, r[nxt], n, this->st, partials, counter, st));
+      TRY2(precondition(rk, z, s));
+      TRY2(launch_rz_dot(rk, z, r[nxt], n, this->st, partials, counter, s));
       if (layout == 0) {
         fa.z = z;
         fa.p_old = p[cur];
         fa.p_new = p[nxt];
         fa.w = w;
-        TRY(launch_p_spmv_flat(fa, st));
+        TRY2(launch_p_spmv_flat(fa, s));
         fa.x_old = x[cur];
         fa.x_new = x[nxt];
         fa.p = p[nxt];
         fa.r = r[nxt];
-        TRY(launch_resid_flat(fa, 0, st));
+        TRY2(launch_resid_flat(fa, 0, s));
       } else {
         ca.z = z;
         ca.p_old = p[cur];
         ca.p_new = p[nxt];
         ca.w = w;
-        TRY(launch_p_spmv_csr(ca, st));
+        TRY2(launch_p_spmv_csr(ca, s));
         ca.x_old = x[cur];
         ca.x_new = x[nxt];
         ca.p = p[nxt];
         ca.r = r[nxt];
-        TRY(launch_resid_csr(ca, 0, st));
+        TRY2(launch_resid_csr(ca, 0, s));
       }
-    } else {
-      // q = P(r); q -= (q.Ap_j / alfa_j) p_j for the kept directions, newest first
-      float* q = z;
-      TRY(precondition(rk, q));
-      const int kept = npsd_count < 2 ? npsd_count : 2;
-      for (int jj = kept - 1; jj >= 0; --jj) {
-        const int slot = (npsd_count - kept + jj) & 1;  // ring position of direction jj
-        TRY(launch_npsd_proj(q, aq_hist[slot], &this->st->alfa[slot], n, this->st, partials, counter, st));
-        TRY(launch_npsd_axpy(q, q_hist[slot], n, this->st, st));
-      }
-      // the new direction replaces the oldest slot
-      const int slot_new = npsd_count & 1;
-      MLPCG_HIP(hipMemcpyAsync(q_hist[slot_new], q, vb, hipMemcpyDeviceToDevice, st));
-      if (layout == 0) {
-        fa.z = q_hist[slot_new];
-        fa.r = const_cast<float*>(rk);
-        fa.w = aq_hist[slot_new];
-        TRY(launch_npsd_spmv_flat(fa, &this->st->alfa[slot_new], st));
-        fa.x_old = x[cur];
-        fa.x_new = x[nxt];
-        fa.p = q_hist[slot_new];
-        fa.r = r[nxt];
-        TRY(launch_resid_flat(fa, 0, st));
-      } else {
-        ca.z = q_hist[slot_new];
-        ca.r = const_cast<float*>(rk);
-        ca.w = aq_hist[slot_new];
-        TRY(launch_npsd_spmv_csr(ca, &this->st->alfa[slot_new], st));
-        ca.x_old = x[cur];
-        ca.x_new = x[nxt];
-        ca.p = q_hist[slot_new];
-        ca.r = r[nxt];
-        TRY(launch_resid_csr(ca, 0, st));
-      }
-      ++npsd_count;
+      return 0;
     }
+    // npsd: q = P(r); q -= (q.Ap_j / alfa_j) p_j for the kept directions, newest first
+    float* q = z;
+    TRY2(precondition(rk, q, s));
+    const int kept = cnt < 2 ? cnt : 2;
+    for (int jj = kept - 1; jj >= 0; --jj) {
+      const int slot = (cnt - kept + jj) & 1;  // ring position of direction jj
+      TRY2(launch_npsd_proj(q, aq_hist[slot], &this->st->alfa[slot], n, this->st, partials, counter, s));
+      TRY2(launch_npsd_axpy(q, q_hist[slot], n, this->st, s));
+    }
+    // the new direction replaces the oldest slot
+    const int slot_new = cnt & 1;
+    MLPCG_HIP(hipMemcpyAsync(q_hist[slot_new], q, vb, hipMemcpyDeviceToDevice, s));
+    if (layout == 0) {
+      fa.z = q_hist[slot_new];
+      fa.r = const_cast<float*>(rk);
+      fa.w = aq_hist[slot_new];
+      TRY2(launch_npsd_spmv_flat(fa, &this->st->alfa[slot_new], s));
+      fa.x_old = x[cur];
+      fa.x_new = x[nxt];
+      fa.p = q_hist[slot_new];
+      fa.r = r[nxt];
+      TRY2(launch_resid_flat(fa, 0, s));
+    } else {
+      ca.z = q_hist[slot_new];
+      ca.r = const_cast<float*>(rk);
+      ca.w = aq_hist[slot_new];
+      TRY2(launch_npsd_spmv_csr(ca, &this->st->alfa[slot_new], s));
+      ca.x_old = x[cur];
+      ca.x_new = x[nxt];
+      ca.p = q_hist[slot_new];
+      ca.r = r[nxt];
+      TRY2(launch_resid_csr(ca, 0, s));
+    }
+    return 0;
+#undef TRY2
+  };
+
+  int k = 0;
+  int npsd_count = 0;  // directions created (npsd)
+  for (k = 1; k <= max_it; ++k) {
+    // iterations k and k+2 launch the same kernels with the same arguments (buffer
+    // parity k & 1; for npsd the direction ring is periodic once two directions exist)
+    const bool replay = use_graphs && (method != kNpsd || npsd_count >= 2);
+    if (replay) {
+      const int par = k & 1;
+      if (!graph_exec[par]) {
+        hipGraph_t g = nullptr;
+        MLPCG_HIP(hipStreamBeginCapture(run, hipStreamCaptureModeThreadLocal));
+        const int rc3 = enqueue_iter(k, npsd_count, run);
+        const hipError_t ce = hipStreamEndCapture(run, &g);
+        if (rc3) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc3;
+        }
+        MLPCG_HIP(ce);
+        const hipError_t ie = hipGraphInstantiate(&graph_exec[par], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        MLPCG_HIP(ie);
+        graph_P = P;
+        graph_b = b;
+        graph_method = method;
+      }
+      MLPCG_HIP(hipGraphLaunch(graph_exec[par], run));
+    } else {
+      TRY(enqueue_iter(k, npsd_count, run));
+    }
+    if (method == kNpsd) ++npsd_count;
     // status of iteration k goes to the host on the side stream
     const int e = k & 3;
-    MLPCG_HIP(hipEventRecord(ev_iter[e], st));
+    MLPCG_HIP(hipEventRecord(ev_iter[e], run));
     MLPCG_HIP(hipStreamWaitEvent(side, ev_iter[e], 0));
     MLPCG_HIP(hipMemcpyAsync(&st_host[e], this->st, sizeof(SolveState), hipMemcpyDeviceToHost, side));
     MLPCG_HIP(hipEventRecord(ev_stat[e], side));
@@ -270,15 +327,15 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
   }
 #undef TRY
   // final state
-  MLPCG_HIP(hipStreamSynchronize(st));
+  MLPCG_HIP(hipStreamSynchronize(run));
   MLPCG_HIP(hipStreamSynchronize(side));
   SolveState fin;
   MLPCG_HIP(hipMemcpy(&fin, this->st, sizeof(SolveState), hipMemcpyDeviceToHost));
   const int iters = fin.iter;
-  MLPCG_HIP(hipMemcpyAsync(x_out, x[iters & 1], vb, hipMemcpyDeviceToDevice, st));
+  MLPCG_HIP(hipMemcpyAsync(x_out, x[iters & 1], vb, hipMemcpyDeviceToDevice, run));
   if (history_out)
     MLPCG_HIP(hipMemcpy(history_out, history, (size_t)(iters + 1) * sizeof(double), hipMemcpyDeviceToHost));
-  MLPCG_HIP(hipStreamSynchronize(st));
+  MLPCG_HIP(hipStreamSynchronize(run));
   if (info) {
     info->iters = iters;
     info->rel_res = fin.norm;
@@ -289,6 +346,16 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
     else info->status = 1;
   }
   return 0;
+}
+
+void Solver::drop_graphs() {
+  for (int i = 0; i < 2; ++i) {
+    if (graph_exec[i]) (void)hipGraphExecDestroy(graph_exec[i]);
+    graph_exec[i] = nullptr;
+  }
+  graph_P = nullptr;
+  graph_b = nullptr;
+  graph_method = -1;
 }
 
 }  // namespace mlpcg

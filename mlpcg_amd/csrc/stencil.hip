@@ -324,15 +324,45 @@ __global__ __launch_bounds__(kThreads) void upsample2_kernel(UpArgs a) {
   const TI* r11 = in + ((long)i1 * m2 + j1) * m3;
   const A hi0 = A(1) - (A)li, hi1 = (A)li, hj0 = A(1) - (A)lj, hj1 = (A)lj;
   A out[VZ];
+  if constexpr (VZ >= 2) {
+    // coarse z segment [k0/2 - 1, k0/2 + VZ/2] of the four rows, indices clamped to the
+    // grid: with the clamp the interior weights reproduce the edge cases exactly
+    // (f = 0 -> x[0]; f = 2m3-1 -> x[m3-1]), so the taps below are compile-time
+    constexpr int NS = VZ / 2 + 2;
+    const int kc = c.k0 >> 1;
+    const int klo = max(kc - 1, 0), khi = min(kc + VZ / 2, m3 - 1);
+    A seg[4][NS];
+    const TI* rows[4] = {r00, r01, r10, r11};
 #pragma unroll
-  for (int t = 0; t < VZ; ++t) {
-    int k0, k1;
-    float lk;
-    up_axis(c.k0 + t, m3, a.nearest, k0, k1, lk);
-    const A w0 = A(1) - (A)lk, w1 = (A)lk;
-    const A v0 = hj0 * (w0 * (A)r00[k0] + w1 * (A)r00[k1]) + hj1 * (w0 * (A)r01[k0] + w1 * (A)r01[k1]);
-    const A v1 = hj0 * (w0 * (A)r10[k0] + w1 * (A)r10[k1]) + hj1 * (w0 * (A)r11[k0] + w1 * (A)r11[k1]);
-    out[t] = (hi0 * v0 + hi1 * v1) * (A)a.scale;
+    for (int q = 0; q < 4; ++q) {
+      load_vec<VZ / 2>(rows[q] + kc, &seg[q][1]);
+      seg[q][0] = (A)rows[q][klo];
+      seg[q][NS - 1] = (A)rows[q][khi];
+    }
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) {
+      A z[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (a.nearest) z[q] = seg[q][t / 2 + 1];
+        else if ((t & 1) == 0) z[q] = A(0.25) * seg[q][t / 2] + A(0.75) * seg[q][t / 2 + 1];
+        else z[q] = A(0.75) * seg[q][t / 2 + 1] + A(0.25) * seg[q][t / 2 + 2];
+      }
+      const A v0 = hj0 * z[0] + hj1 * z[1];
+      const A v1 = hj0 * z[2] + hj1 * z[3];
+      out[t] = (hi0 * v0 + hi1 * v1) * (A)a.scale;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) {
+      int k0, k1;
+      float lk;
+      up_axis(c.k0 + t, m3, a.nearest, k0, k1, lk);
+      const A w0 = A(1) - (A)lk, w1 = (A)lk;
+      const A v0 = hj0 * (w0 * (A)r00[k0] + w1 * (A)r00[k1]) + hj1 * (w0 * (A)r01[k0] + w1 * (A)r01[k1]);
+      const A v1 = hj0 * (w0 * (A)r10[k0] + w1 * (A)r10[k1]) + hj1 * (w0 * (A)r11[k0] + w1 * (A)r11[k1]);
+      out[t] = (hi0 * v0 + hi1 * v1) * (A)a.scale;
+    }
   }
   const long off = b * Vf + c.v0;
   if (a.prev) {
@@ -525,9 +555,10 @@ int launch_avgpool2(int dtype, const void* in, void* out, int B, int n1, int n2,
 
 template <typename TI, typename TO>
 static void launch_up_t(const UpArgs& a, hipStream_t st) {
-  const int vz = pick_vz(a.n3, 4);
+  const int vz = pick_vz(a.n3, 8);
   const long items = (long)a.n1 * a.n2 * (a.n3 / vz);
-  if (vz == 4) upsample2_kernel<TI, TO, TO, 4><<<grid1(items, a.B), kThreads, 0, st>>>(a);
+  if (vz == 8) upsample2_kernel<TI, TO, TO, 8><<<grid1(items, a.B), kThreads, 0, st>>>(a);
+  else if (vz == 4) upsample2_kernel<TI, TO, TO, 4><<<grid1(items, a.B), kThreads, 0, st>>>(a);
   else if (vz == 2) upsample2_kernel<TI, TO, TO, 2><<<grid1(items, a.B), kThreads, 0, st>>>(a);
   else upsample2_kernel<TI, TO, TO, 1><<<grid1(items, a.B), kThreads, 0, st>>>(a);
 }

@@ -56,7 +56,7 @@ class FusedPCG:
     cells, A given as a scipy CSR matrix in fluid order).
     """
 
-    def __init__(self, flags, layout="flags", A=None, device="cuda"):
+    def __init__(self, flags, layout="flags", A=None, device="cuda", graphs=False):
         self.device = torch.device(device)
         flags_np = np.asarray(flags)
         self.shape = tuple(int(s) for s in flags_np.shape)
@@ -84,6 +84,12 @@ class FusedPCG:
         else:
             raise ValueError(f"unknown layout {layout}")
         self._h = h
+        self.set_graphs(graphs)
+
+    def set_graphs(self, enable: bool):
+        """Replay each iteration as a captured HIP graph (bit-identical results)."""
+        check(lib.mlpcg_solver_set_graphs(self._h, int(bool(enable))), "solver_set_graphs")
+        self.graphs = bool(enable)
 
     def vector(self, values):
         """Device float32 vector in this solver's layout from fluid-cell values."""

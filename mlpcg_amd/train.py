@@ -29,17 +29,21 @@ MODELS = {"sm": lambda n, c: SmallSMModelDn3D(n, c, "trilinear"),
 
 
 class LaplacianFunction(torch.autograd.Function):
-    """y = A x on the full grid (A symmetric, so the backward is A g).  x: (B, V) float32."""
+    """y = M A M x on the full grid, M the restriction to the fluid cells (the matrix-free
+    kernel reads neighbours as if x vanished off the fluid, so the input is masked);
+    the map is symmetric, so the backward is the same operator.  x: (B, V) float32."""
 
     @staticmethod
     def forward(ctx, x, code):
         ctx.save_for_backward(code)
-        return torch.stack([K.spmv_flags(code, xi.contiguous()) for xi in x])
+        m = (code != 0).reshape(1, -1)
+        return torch.stack([K.spmv_flags(code, xi.contiguous()) for xi in x * m])
 
     @staticmethod
     def backward(ctx, g):
         code, = ctx.saved_tensors
-        return torch.stack([K.spmv_flags(code, gi.contiguous()) for gi in g]), None
+        m = (code != 0).reshape(1, -1)
+        return torch.stack([K.spmv_flags(code, gi.contiguous()) for gi in g * m]), None
 
 
 class SceneBatch:

@@ -153,3 +153,29 @@ def test_graph_replay_matches_direct_launches(dev, method, layout):
         assert i1["iters"] == i2["iters"]
         assert np.array_equal(i1["history"], i2["history"])
         assert torch.equal(x1, x2)
+
+
+@pytest.mark.parametrize("method", ["npcg", "npcg_flex", "npsd"])
+def test_dropin_loops_with_model_predict(dev, method):
+    """The reference-signature loops (cg_tests.py) on torch tensors with the HIP CSR
+    SpMV and test.py's predict follow the native driver's residual history."""
+    from mlpcg_amd import solver as SV
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    sc, A, b = _scene(n=32, seed=10)
+    P = M.init_params(3, 1, torch.Generator().manual_seed(4))
+    m = SmallSMModelDn3D(3, 1).to(dev)
+    m.load_state_dict({k: v.float() for k, v in P.items()})
+    img = torch.as_tensor(S.image_channels(sc.flags, 1), device=dev)
+    fluid = torch.as_tensor(sc.fluid_cells, device=dev)
+    predict = SV.model_predict(m, img, fluid)
+    Ad = SV.CSRMatrix(A, dev)
+    bt = torch.as_tensor(b, dtype=torch.float32, device=dev)
+    hist = []
+    f = {"npcg": SV.npcg, "npcg_flex": SV.npcg_flex, "npsd": SV.npsd}[method]
+    x, k, timer, norm = f(bt, Ad, torch.zeros_like(bt), predict, 8, tol=1e-12, callback=hist.append)
+    assert k == 8 and len(hist) == 8
+    s = SV.FusedPCG(sc.flags, "csr", A=A, device=dev)
+    _, info = s.solve(s.vector(b), m.make_preconditioner(img), method=method, max_it=8, tol=1e-12,
+                      history=True)
+    assert _hist_close(hist, info["history"][1:], 8, 1e-3) < 5e-3
+    assert "Total" in timer.top_level_clocks

@@ -40,7 +40,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--levels", type=int, default=5)
-    ap.add_argument("--method", default="npcg")
+    ap.add_argument("--method", default="npsd",
+                    help="solver of the benchmark value (npsd: the reference's default, test.py:327)")
+    ap.add_argument("--also", default="npcg_flex", help="comma list of further methods reported")
     ap.add_argument("--max-it", type=int, default=300)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -330,28 +332,37 @@ def main():
     if not args.skip_fp16:
         plans["fp16"] = model.make_preconditioner(image, torch.float16)
 
-    def step(name):
+    def step(name, method):
         plans[name].update(image)
-        return solver.solve(b, plans[name], method=args.method, max_it=args.max_it, tol=args.tol)
+        return solver.solve(b, plans[name], method=method, max_it=args.max_it, tol=args.tol)
 
-    results = {}
-    for name in plans:
-        for _ in range(args.warmup):
-            step(name)
-        k = args.steps if name in ("bf16", "fp32") else max(1, args.steps // 2)
-        times, infos = [], []
-        xs = None
-        for _ in range(k):
-            t, (xs, info) = timed(lambda: step(name))
-            times.append(t)
-            infos.append(info)
-        t_setup, _ = timed(lambda: plans[name].update(image))
-        results[name] = {"s_per_solve": float(np.mean(times)), "min_s": float(np.min(times)),
+    def run_method(method, names):
+        out = {}
+        for name in names:
+            for _ in range(args.warmup):
+                step(name, method)
+            k = args.steps if name in ("bf16", "fp32") else max(1, args.steps // 2)
+            times, infos = [], []
+            xs = None
+            for _ in range(k):
+                t, (xs, info) = timed(lambda: step(name, method))
+                times.append(t)
+                infos.append(info)
+            t_setup, _ = timed(lambda: plans[name].update(image))
+            out[name] = {"s_per_solve": float(np.mean(times)), "min_s": float(np.min(times)),
                          "iters": infos[-1]["iters"], "status": infos[-1]["status"],
-                         "rel_res_device_fp64": infos[-1]["rel_res"], "setup_s": t_setup,
-                         "x": xs}
-        log(f"[bench] {name}: {results[name]['s_per_solve'] * 1e3:.2f} ms/solve, "
-            f"{results[name]['iters']} iterations, status {results[name]['status']}")
+                         "rel_res_device_fp64": infos[-1]["rel_res"], "setup_s": t_setup, "x": xs}
+            log(f"[bench] {method} {name}: {out[name]['s_per_solve'] * 1e3:.2f} ms/solve, "
+                f"{out[name]['iters']} iterations, status {out[name]['status']}")
+        return out
+
+    results = run_method(args.method, list(plans))
+    others = {}
+    for m in [t for t in args.also.split(",") if t and t != args.method]:
+        o = run_method(m, [n for n in ("bf16", "fp32") if n in plans])
+        for v in o.values():
+            v.pop("x")
+        others[m] = {k: {kk: vv for kk, vv in v.items()} for k, v in o.items()}
 
     # FP64 check of the returned iterates on the host
     A = S.build_matrix(sc.flags)
@@ -417,6 +428,7 @@ def main():
         "iters": {k: v["iters"] for k, v in results.items()},
         "iter_ratio_bf16_vs_fp32": round(bf["iters"] / max(f32["iters"], 1), 3),
         "solves": results,
+        "other_methods": others,
         "train": train,
     }
     print(json.dumps(out), flush=True)

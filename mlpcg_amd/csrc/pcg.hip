@@ -15,9 +15,10 @@
 
 namespace mlpcg {
 
-constexpr int kMaxRedBlocks = 2048;
-
-__device__ __forceinline__ int grid_blocks_needed() { return gridDim.x; }
+// Reduction kernels run up to kMaxRedBlocks blocks (one z-run per thread at 256^3, so
+// enough bytes are in flight to cover HBM latency); their partials are [NR][blocks].
+constexpr int kMaxRedBlocks = 16384;
+constexpr int kDotRun = 16;  // floats per thread per vector in rz_dot
 
 // ---------------------------------------------------------------------------
 // code from flags
@@ -207,24 +208,26 @@ __global__ __launch_bounds__(kThreads) void rz_dot_kernel(const float* __restric
                                                           unsigned* counter) {
   if (st->done) return;
   double red[2] = {0.0, 0.0};
-  const long n4 = n / 4;
+  // each thread owns runs of kDotRun consecutive floats, all loads issued before use so
+  // every wave keeps 3 x 4 KB in flight
+  const long nr = n / kDotRun;
   const bool flex = (st->method == kNpcgFlex) && r_old != nullptr;
-  for (long q = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; q < n4 + (n & 3);
+  for (long q = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; q < nr;
        q += (long)gridDim.x * kThreads) {
-    if (q < n4) {
-      float rv[4], zv[4];
-      load_vec<4>(r + 4 * q, rv);
-      load_vec<4>(z + 4 * q, zv);
+    float rv[kDotRun], zv[kDotRun], ov[kDotRun];
+    load_vec<kDotRun>(r + kDotRun * q, rv);
+    load_vec<kDotRun>(z + kDotRun * q, zv);
+    if (flex) load_vec<kDotRun>(r_old + kDotRun * q, ov);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) red[0] += (double)rv[t] * (double)zv[t];
-      if (flex) {
-        float ov[4];
-        load_vec<4>(r_old + 4 * q, ov);
+    for (int t = 0; t < kDotRun; ++t) red[0] += (double)rv[t] * (double)zv[t];
+    if (flex) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) red[1] += (double)zv[t] * ((double)rv[t] - (double)ov[t]);
-      }
-    } else {
-      const long e = 4 * n4 + (q - n4);
+      for (int t = 0; t < kDotRun; ++t) red[1] += (double)zv[t] * ((double)rv[t] - (double)ov[t]);
+    }
+  }
+  // tail (n not a multiple of kDotRun): first block only
+  if (blockIdx.x == 0) {
+    for (long e = nr * kDotRun + threadIdx.x; e < n; e += kThreads) {
       red[0] += (double)r[e] * (double)z[e];
       if (flex) red[1] += (double)z[e] * ((double)r[e] - (double)r_old[e]);
     }
@@ -503,7 +506,7 @@ int launch_npsd_spmv_flat(const FlatArgs& a, double* alfa_out, hipStream_t st) {
 
 int launch_rz_dot(const float* r, const float* z, const float* r_old, long n, SolveState* s,
                   double* partials, unsigned* counter, hipStream_t st) {
-  rz_dot_kernel<<<red_blocks(n / 4 + 1), kThreads, 0, st>>>(r, z, r_old, n, s, partials, counter);
+  rz_dot_kernel<<<red_blocks(n / kDotRun + 1), kThreads, 0, st>>>(r, z, r_old, n, s, partials, counter);
   return check_launch("rz_dot");
 }
 

@@ -73,17 +73,19 @@ template <typename A>
 __global__ __launch_bounds__(kThreads) void mean_patch_final_kernel(
     const double* __restrict__ partials, int nblk, int NV, double invV, const A* __restrict__ w,
     const A* __restrict__ b, A* __restrict__ z, A* __restrict__ ybar) {
+  // every output is summed by the whole block (strided over the partials, then a block
+  // reduction), so the pass costs nblk/256 loads per thread per output
   __shared__ double sh[4];
   double contrib = 0.0;
-  for (int o = threadIdx.x; o < NV; o += kThreads) {
+  for (int o = 0; o < NV; ++o) {
     double s = 0.0;
-    for (int blk = 0; blk < nblk; ++blk) s += partials[(size_t)blk * NV + o];
+    for (int blk = threadIdx.x; blk < nblk; blk += kThreads) s += partials[(size_t)blk * NV + o];
+    s = block_sum(s, sh);
     const A yb = (A)(s * invV);
-    ybar[o] = yb;
+    if (threadIdx.x == 0) ybar[o] = yb;
     contrib += (double)w[o] * (double)yb;
   }
-  const double tot = block_sum(contrib, sh);
-  if (threadIdx.x == 0) z[0] = (A)(tot + (double)b[0]);
+  if (threadIdx.x == 0) z[0] = (A)(contrib + (double)b[0]);
 }
 
 static int mean_patch_blocks(int n1, int n2, int n3, int vz) {
@@ -281,11 +283,15 @@ __global__ __launch_bounds__(kThreads) void weight_grad_final_kernel(const A* __
                                                                      int nblk, int C,
                                                                      A* __restrict__ gW,
                                                                      A* __restrict__ gb) {
+  // one block per output; the block sums the nblk partials (strided, then block-reduced)
+  __shared__ double sh[4];
   const int NQ = 27 * C + 1, NO = 27 * NQ;
-  const int o = blockIdx.x * kThreads + threadIdx.x;
+  const int o = blockIdx.x;
   if (o >= NO) return;
   double s = 0.0;
-  for (int blk = 0; blk < nblk; ++blk) s += (double)partials[(size_t)blk * NO + o];
+  for (int blk = threadIdx.x; blk < nblk; blk += kThreads) s += (double)partials[(size_t)blk * NO + o];
+  s = block_sum(s, sh);
+  if (threadIdx.x != 0) return;
   const int p = o / NQ, q = o % NQ;
   if (q < 27 * C) gW[p * 27 * C + q] = (A)s;
   else gb[p] = (A)s;
@@ -337,7 +343,7 @@ int launch_weight_grad(int dtype, const void* a, const void* c, const void* img,
     default: MLPCG_REQUIRE(false, "weight_grad: bad dtype %d", dtype);
   }
   const int NO = 27 * (27 * C + 1);
-  const int fb = (NO + kThreads - 1) / kThreads;
+  const int fb = NO;
   if (dtype == kF64)
     weight_grad_final_kernel<double><<<fb, kThreads, 0, st>>>((const double*)workspace, nblk, C,
                                                               (double*)gW, (double*)gb);

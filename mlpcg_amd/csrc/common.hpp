@@ -125,15 +125,14 @@ __device__ __forceinline__ A block_sum(A v, A* sh) {
   return r;
 }
 
-// Deterministic grid reduction without global atomics on the data: every block writes
-// its NR partial sums, the last block to arrive (ticket counter) adds the partials in
-// block order and runs `fin` on the totals, then re-arms the counter for the next use.
-// partials: [NR][gridDim.x] doubles.  Must be called by all threads of the block.
-template <int NR, typename Fin>
-__device__ __forceinline__ void grid_reduce(double (&v)[NR], double* __restrict__ partials,
-                                            unsigned* __restrict__ counter, Fin fin) {
+// Deterministic two-kernel reduction: each block writes its NR partial sums with plain
+// stores (partials: [NR][gridDim.x] doubles); a following one-block kernel adds them in
+// block order.  No device-scope fences or same-address atomics: on a multi-XCD part an
+// agent-scope release writes back the XCD's L2, and thousands of them per kernel cost
+// more than the extra launch.  Must be called by all threads of the block.
+template <int NR>
+__device__ __forceinline__ void block_partials(const double (&v)[NR], double* __restrict__ partials) {
   __shared__ double sh[NR][4];
-  __shared__ bool last;
   const int w = threadIdx.x >> 6;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -141,41 +140,38 @@ __device__ __forceinline__ void grid_reduce(double (&v)[NR], double* __restrict_
     if ((threadIdx.x & 63) == 0) sh[r][w] = s;
   }
   __syncthreads();
-  const unsigned nb = gridDim.x;
   if (threadIdx.x == 0) {
+    const unsigned nb = gridDim.x;
 #pragma unroll
     for (int r = 0; r < NR; ++r)
       partials[(size_t)r * nb + blockIdx.x] = sh[r][0] + sh[r][1] + sh[r][2] + sh[r][3];
-    __threadfence();
-    unsigned ticket = atomicAdd(counter, 1u);
-    last = (ticket == nb - 1);
   }
+}
+
+// Sum of partials[r * nb + 0 .. nb) by one block of kFinThreads threads, in a fixed
+// order (per-thread strided sums with 4 independent accumulators, then a tree over the
+// threads); the result is valid in thread 0.
+constexpr int kFinThreads = 1024;
+
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, unsigned nb) {
+  __shared__ double sh[kFinThreads / 64];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  unsigned b = threadIdx.x;
+  for (; b + 3 * kFinThreads < nb; b += 4 * kFinThreads) {
+    a0 += p[b];
+    a1 += p[b + kFinThreads];
+    a2 += p[b + 2 * kFinThreads];
+    a3 += p[b + 3 * kFinThreads];
+  }
+  for (; b < nb; b += kFinThreads) a0 += p[b];
+  double s = wave_sum((a0 + a1) + (a2 + a3));
   __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double tot[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    double s = 0.0;
-    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
-      s += __builtin_nontemporal_load(&partials[(size_t)r * nb + b]);
-    tot[r] = s;
-  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    double s = wave_sum(tot[r]);
-    if ((threadIdx.x & 63) == 0) sh[r][w] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) t[r] = sh[r][0] + sh[r][1] + sh[r][2] + sh[r][3];
-    fin(t);
-    *counter = 0u;
-    __threadfence();
-  }
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kFinThreads / 64; ++w) t += sh[w];
+  return t;
 }
 
 // Early exit flag shared by all kernels of a running solve: once the solve has

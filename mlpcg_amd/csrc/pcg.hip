@@ -144,25 +144,7 @@ __global__ __launch_bounds__(kThreads) void resid_flat_kernel(FlatArgs a, int in
     store_vec<VZ>(a.r + r.v0, rv);
     if (!init) store_vec<VZ>(a.x_new + r.v0, xn);
   }
-  grid_reduce<2>(red, a.partials, a.counter, [&](const double* t) {
-    if (init) {
-      st->norm_b = sqrt(t[1]);
-      st->iter = 0;
-      st->done = (t[0] == 0.0) ? 1 : 0;
-    } else {
-      st->iter += 1;
-      st->rho_old = st->rho;
-    }
-    st->rr = t[0];
-    st->norm = sqrt(t[0]) / st->norm_b;
-    st->scale = t[0] > 0.0 ? 1.0 / sqrt(t[0]) : 0.0;
-    if (!init) {
-      const double tol = fmax(st->tol, st->atol / st->norm_b);
-      const bool cont = (st->method == kNpsd) ? !(st->norm < tol) : (st->norm > tol);
-      st->done = (cont && st->iter < st->max_it) ? 0 : 1;
-    }
-    if (st->history && st->iter <= st->max_it) st->history[st->iter] = st->norm;
-  });
+  block_partials<2>(red, a.partials);
 }
 
 // p_new = z + beta p_old; w = A p_new; reduces p_new . w -> alpha = rho / (p.w).
@@ -194,10 +176,7 @@ __global__ __launch_bounds__(kThreads) void p_spmv_flat_kernel(FlatArgs a) {
     store_vec<VZ>(a.p_new + r.v0, pn);
     store_vec<VZ>(a.w + r.v0, wv);
   }
-  grid_reduce<1>(red, a.partials, a.counter, [&](const double* t) {
-    st->pw = t[0];
-    st->alpha = st->rho / t[0];
-  });
+  block_partials<1>(red, a.partials);
 }
 
 // rho = r.z (and z.(r - r_old) for the flexible variant) -> beta.
@@ -232,11 +211,7 @@ __global__ __launch_bounds__(kThreads) void rz_dot_kernel(const float* __restric
       if (flex) red[1] += (double)z[e] * ((double)r[e] - (double)r_old[e]);
     }
   }
-  grid_reduce<2>(red, partials, counter, [&](const double* t) {
-    st->rho = t[0];
-    if (st->iter == 0) st->beta = 0.0;
-    else st->beta = (flex ? t[1] : t[0]) / st->rho_old;
-  });
+  block_partials<2>(red, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -278,10 +253,7 @@ __global__ __launch_bounds__(kThreads) void p_spmv_csr_kernel(CsrArgs a) {
     a.w[row] = w;
     red[0] += (double)pn * (double)w;
   }
-  grid_reduce<1>(red, a.partials, a.counter, [&](const double* t) {
-    st->pw = t[0];
-    st->alpha = st->rho / t[0];
-  });
+  block_partials<1>(red, a.partials);
 }
 
 __global__ __launch_bounds__(kThreads) void resid_csr_kernel(CsrArgs a, int init) {
@@ -306,25 +278,7 @@ __global__ __launch_bounds__(kThreads) void resid_csr_kernel(CsrArgs a, int init
     red[0] += res * res;
     red[1] += bv * bv;
   }
-  grid_reduce<2>(red, a.partials, a.counter, [&](const double* t) {
-    if (init) {
-      st->norm_b = sqrt(t[1]);
-      st->iter = 0;
-      st->done = (t[0] == 0.0) ? 1 : 0;
-    } else {
-      st->iter += 1;
-      st->rho_old = st->rho;
-    }
-    st->rr = t[0];
-    st->norm = sqrt(t[0]) / st->norm_b;
-    st->scale = t[0] > 0.0 ? 1.0 / sqrt(t[0]) : 0.0;
-    if (!init) {
-      const double tol = fmax(st->tol, st->atol / st->norm_b);
-      const bool cont = (st->method == kNpsd) ? !(st->norm < tol) : (st->norm > tol);
-      st->done = (cont && st->iter < st->max_it) ? 0 : 1;
-    }
-    if (st->history && st->iter <= st->max_it) st->history[st->iter] = st->norm;
-  });
+  block_partials<2>(red, a.partials);
 }
 
 // Plain y = A x (CSR, FP32) and y = A x (flat, FP32) for the drop-in operator.
@@ -377,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void dot_kernel(const float* __restrict__
   double red[1] = {0.0};
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
     red[0] += (double)x[i] * (double)y[i];
-  grid_reduce<1>(red, partials, counter, [&](const double* t) { out[0] = t[0]; });
+  block_partials<1>(red, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -396,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void npsd_proj_dot_kernel(const float* __
   for (long i = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; i < n;
        i += (long)gridDim.x * kThreads)
     red[0] += (double)q[i] * (double)Ap[i];
-  grid_reduce<1>(red, partials, counter, [&](const double* t) { st->proj = t[0] / alfa_j[0]; });
+  block_partials<1>(red, partials);
 }
 
 // q -= proj * p_j
@@ -435,10 +389,7 @@ __global__ __launch_bounds__(kThreads) void npsd_spmv_flat_kernel(FlatArgs a, do
     }
     store_vec<VZ>(a.w + r.v0, wv);
   }
-  grid_reduce<2>(red, a.partials, a.counter, [&](const double* t) {
-    alfa_out[0] = t[0];
-    st->alpha = t[1] / t[0];
-  });
+  block_partials<2>(red, a.partials);
 }
 
 __global__ __launch_bounds__(kThreads) void npsd_spmv_csr_kernel(CsrArgs a, double* alfa_out) {
@@ -453,10 +404,72 @@ __global__ __launch_bounds__(kThreads) void npsd_spmv_csr_kernel(CsrArgs a, doub
     red[0] += (double)a.z[row] * (double)w;
     red[1] += (double)a.z[row] * (double)a.r[row];
   }
-  grid_reduce<2>(red, a.partials, a.counter, [&](const double* t) {
-    alfa_out[0] = t[0];
-    st->alpha = t[1] / t[0];
-  });
+  block_partials<2>(red, a.partials);
+}
+
+
+// ---------------------------------------------------------------------------
+// Finalisation of the two-kernel reductions: one block sums the partials in block order
+// and updates the solver scalars.
+// ---------------------------------------------------------------------------
+enum FinOp : int { kFinResidInit = 0, kFinResid, kFinPW, kFinRZ, kFinDot, kFinProj, kFinNpsd };
+
+__global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
+                                                               const double* __restrict__ partials,
+                                                               unsigned nb, SolveState* st,
+                                                               double* aux) {
+  if (op != kFinResidInit && op != kFinDot && st->done) return;  // uniform over the block
+  double t[2] = {0.0, 0.0};
+  for (int r = 0; r < nr; ++r) t[r] = sum_partials(partials + (size_t)r * nb, nb);
+  if (threadIdx.x != 0) return;
+  switch (op) {
+    case kFinResidInit:
+    case kFinResid: {
+      if (op == kFinResidInit) {
+        st->norm_b = sqrt(t[1]);
+        st->iter = 0;
+        st->done = (t[0] == 0.0) ? 1 : 0;
+      } else {
+        st->iter += 1;
+        st->rho_old = st->rho;
+      }
+      st->rr = t[0];
+      st->norm = sqrt(t[0]) / st->norm_b;
+      st->scale = t[0] > 0.0 ? 1.0 / sqrt(t[0]) : 0.0;
+      if (op == kFinResid) {
+        const double tol = fmax(st->tol, st->atol / st->norm_b);
+        const bool cont = (st->method == kNpsd) ? !(st->norm < tol) : (st->norm > tol);
+        st->done = (cont && st->iter < st->max_it) ? 0 : 1;
+      }
+      if (st->history && st->iter <= st->max_it) st->history[st->iter] = st->norm;
+      break;
+    }
+    case kFinPW:
+      st->pw = t[0];
+      st->alpha = st->rho / t[0];
+      break;
+    case kFinRZ: {
+      const bool flex = (st->method == kNpcgFlex);
+      st->rho = t[0];
+      st->beta = (st->iter == 0) ? 0.0 : (flex ? t[1] : t[0]) / st->rho_old;
+      break;
+    }
+    case kFinDot:
+      aux[0] = t[0];
+      break;
+    case kFinProj:
+      st->proj = t[0] / aux[0];
+      break;
+    case kFinNpsd:
+      aux[0] = t[0];
+      st->alpha = t[1] / t[0];
+      break;
+  }
+}
+
+static void finalize(int op, int nr, double* partials, unsigned nb, SolveState* st, double* aux,
+                     hipStream_t s) {
+  finalize_kernel<<<1, kFinThreads, 0, s>>>(op, nr, partials, nb, st, aux);
 }
 
 // ---------------------------------------------------------------------------
@@ -479,6 +492,11 @@ int launch_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3
   return check_launch("flags_code");
 }
 
+static unsigned flat_blocks(const FlatArgs& a) {
+  const int vz = flat_vz(a.n3);
+  return red_blocks((long)a.n1 * a.n2 * (a.n3 / vz));
+}
+
 #define MLPCG_FLAT_DISPATCH(KERN, ITEMS_VZ, ...)                                                \
   do {                                                                                          \
     const int vz_ = flat_vz(a.n3);                                                              \
@@ -491,22 +509,27 @@ int launch_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3
 
 int launch_resid_flat(const FlatArgs& a, int init, hipStream_t st) {
   MLPCG_FLAT_DISPATCH(resid_flat_kernel, 0, a, init);
+  finalize(init ? kFinResidInit : kFinResid, 2, a.partials, flat_blocks(a), a.st, nullptr, st);
   return check_launch("resid_flat");
 }
 
 int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st) {
   MLPCG_FLAT_DISPATCH(p_spmv_flat_kernel, 0, a);
+  finalize(kFinPW, 1, a.partials, flat_blocks(a), a.st, nullptr, st);
   return check_launch("p_spmv_flat");
 }
 
 int launch_npsd_spmv_flat(const FlatArgs& a, double* alfa_out, hipStream_t st) {
   MLPCG_FLAT_DISPATCH(npsd_spmv_flat_kernel, 0, a, alfa_out);
+  finalize(kFinNpsd, 2, a.partials, flat_blocks(a), a.st, alfa_out, st);
   return check_launch("npsd_spmv_flat");
 }
 
 int launch_rz_dot(const float* r, const float* z, const float* r_old, long n, SolveState* s,
                   double* partials, unsigned* counter, hipStream_t st) {
-  rz_dot_kernel<<<red_blocks(n / kDotRun + 1), kThreads, 0, st>>>(r, z, r_old, n, s, partials, counter);
+  const unsigned nb = red_blocks(n / kDotRun + 1);
+  rz_dot_kernel<<<nb, kThreads, 0, st>>>(r, z, r_old, n, s, partials, counter);
+  finalize(kFinRZ, 2, partials, nb, s, nullptr, st);
   return check_launch("rz_dot");
 }
 
@@ -523,23 +546,31 @@ int launch_gather(const int* idx, const float* grid, float* dst, long n, const i
 }
 
 int launch_p_spmv_csr(const CsrArgs& a, hipStream_t st) {
-  p_spmv_csr_kernel<<<red_blocks(a.n), kThreads, 0, st>>>(a);
+  const unsigned nb = red_blocks(a.n);
+  p_spmv_csr_kernel<<<nb, kThreads, 0, st>>>(a);
+  finalize(kFinPW, 1, a.partials, nb, a.st, nullptr, st);
   return check_launch("p_spmv_csr");
 }
 
 int launch_resid_csr(const CsrArgs& a, int init, hipStream_t st) {
-  resid_csr_kernel<<<red_blocks(a.n), kThreads, 0, st>>>(a, init);
+  const unsigned nb = red_blocks(a.n);
+  resid_csr_kernel<<<nb, kThreads, 0, st>>>(a, init);
+  finalize(init ? kFinResidInit : kFinResid, 2, a.partials, nb, a.st, nullptr, st);
   return check_launch("resid_csr");
 }
 
 int launch_npsd_spmv_csr(const CsrArgs& a, double* alfa_out, hipStream_t st) {
-  npsd_spmv_csr_kernel<<<red_blocks(a.n), kThreads, 0, st>>>(a, alfa_out);
+  const unsigned nb = red_blocks(a.n);
+  npsd_spmv_csr_kernel<<<nb, kThreads, 0, st>>>(a, alfa_out);
+  finalize(kFinNpsd, 2, a.partials, nb, a.st, alfa_out, st);
   return check_launch("npsd_spmv_csr");
 }
 
 int launch_npsd_proj(const float* q, const float* Ap, const double* alfa_j, long n, SolveState* s,
                      double* partials, unsigned* counter, hipStream_t st) {
-  npsd_proj_dot_kernel<<<red_blocks(n), kThreads, 0, st>>>(q, Ap, alfa_j, n, s, partials, counter);
+  const unsigned nb = red_blocks(n);
+  npsd_proj_dot_kernel<<<nb, kThreads, 0, st>>>(q, Ap, alfa_j, n, s, partials, counter);
+  finalize(kFinProj, 1, partials, nb, s, const_cast<double*>(alfa_j), st);
   return check_launch("npsd_proj");
 }
 
@@ -574,7 +605,9 @@ int launch_spmv_flat(const uint8_t* code, const float* x, float* y, int n1, int 
 
 int launch_dot(const float* x, const float* y, long n, double* partials, unsigned* counter,
                double* out, hipStream_t st) {
-  dot_kernel<<<red_blocks(n), kThreads, 0, st>>>(x, y, n, partials, counter, out);
+  const unsigned nb = red_blocks(n);
+  dot_kernel<<<nb, kThreads, 0, st>>>(x, y, n, partials, counter, out);
+  finalize(kFinDot, 1, partials, nb, nullptr, out, st);
   return check_launch("dot");
 }
 

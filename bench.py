@@ -5,9 +5,14 @@ SM preconditioner, plus Adam train steps/s of the preconditioner.
 One step = one full solve of the 256^3 pillars scene (24 vertical solid pillars of
 radius 4-10, solid walls, air above z = 0.55 N) with the l=5 SmallSMModelDn3D
 preconditioner (C=1 image diag(A)/6): the image-dependent setup (pyramid, stencils,
-SMLinear coefficients) followed by npcg to ||b - Ax||/||b|| <= 1e-6, the residual
-recomputed in FP64 every iteration.  `value` is the mean seconds per step with the BF16
-preconditioner; inputs are resident in HBM before timing.
+SMLinear coefficients) followed by npsd (or --method) to ||b - Ax||/||b|| <= 1e-6, the
+residual recomputed in FP64 every iteration.  `value` is the mean seconds per step with
+the BF16 preconditioner; inputs are resident in HBM before timing.
+
+roofline.traffic is the HBM bytes per launch of the dominant kernel read by two
+rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, calibrated on a 1 GiB copy); the
+passes are recorded in profiles/sm_apply_bf16_level0_pillars256_hbm_traffic.txt and
+apply to the 256^3 workload only (null otherwise).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -31,6 +36,19 @@ DEFAULT_WEIGHTS = os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c1.pt")
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_RECORD = os.path.join(ROOT, "profiles", "sm_apply_bf16_level0_pillars256_hbm_traffic.txt")
+
+
+def recorded_traffic(n):
+    """HBM bytes per launch of the level-0 BF16 apply at 256^3 from the counter passes."""
+    if n != 256 or not os.path.exists(TRAFFIC_RECORD):
+        return None
+    for line in open(TRAFFIC_RECORD):
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(line.split("=")[1])
+    return None
 
 
 def parse():
@@ -385,7 +403,7 @@ def main():
     achieved = alg / t_k / 1e9
     copy_gbs = copy_bandwidth(dev)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": recorded_traffic(args.n),
             "kernel": "sm_apply_kernel<bf16,f32,bf16,bf16,VZ=8> level 0",
             "avg_ms": round(t_k * 1e3, 4), "algorithmic_bytes": alg,
             "measured_copy_GBs": round(copy_gbs, 1)}

@@ -310,6 +310,12 @@ void mlpcg_precond_destroy(mlpcg_precond* P) {
   delete P;
 }
 
+int mlpcg_set_fused_apply(int enable) {
+  const int prev = fused_apply_enabled();
+  set_fused_apply(enable);
+  return prev;
+}
+
 // --------------------------------------------------------------------------- solver
 int mlpcg_solver_create_flags(mlpcg_solver** out, const int8_t* flags, int n1, int n2, int n3,
                               void* stream) {

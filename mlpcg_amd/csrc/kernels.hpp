@@ -37,6 +37,30 @@ struct UpArgs {
   int B, n1, n2, n3;
 };
 
+// Fused apply for 16-bit plans (fused.hip): K recomputed on the matrix cores from the
+// 16-bit image img [C][V] and the packed weights (launch_fused_pack), applied to x with the
+// same epilogue as ApplyArgs.  Grids with n1, n2 % 8 == 0 and n3 % 64 == 0, B = 1, C <= 2.
+struct FusedArgs {
+  const void* img;
+  const void* apack;
+  const float* binit;
+  const void* x;
+  void* y;
+  const void* prev;
+  const void* c0;
+  const void* c1;
+  const uint8_t* mask_a;
+  const uint8_t* mask_b;
+  const double* in_scale;
+  const int* done;
+  int n1, n2, n3;
+};
+
+size_t fused_weights_bytes(int C);
+bool fused_eligible(int C, int B, int n1, int n2, int n3);
+int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st);
+int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream_t st);
+
 int launch_stencil_weights(int dtype, const void* img, const void* W, const void* b, void* K, int C,
                            int n1, int n2, int n3, bool trans, hipStream_t st);
 int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& a, hipStream_t st);

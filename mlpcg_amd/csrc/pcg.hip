@@ -7,9 +7,10 @@
 // the grid counts as solid.  Then for a fluid v:  (A x)(v) = A_vv x(v) - sum_nbr x(nbr),
 // the sum running over all in-grid face neighbours (x vanishes on non-fluid cells).
 //
-// Every reduction is the deterministic last-block pattern of common.hpp; its finalizer
-// updates the solver scalars in SolveState on the device, so an iteration needs no host
-// round trip.  Every kernel returns at once when state->done is set.
+// Every reduction is two kernels (block partials, then a one-block finalize that adds
+// them in block order, see common.hpp); the finalize updates the solver scalars in
+// SolveState on the device, so an iteration needs no host round trip.  Every kernel
+// returns at once when state->done is set.
 #include "common.hpp"
 #include "pcg.hpp"
 
@@ -335,78 +336,164 @@ __global__ __launch_bounds__(kThreads) void dot_kernel(const float* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// npsd (cg_tests.py:217-281): projections against the last directions, then the
-// A-product and the coefficient q.r / q.Aq.
+// npsd (cg_tests.py:217-281).  With N the newest kept direction and O the older one,
+// the reference projects q = P(r) newest first:
+//   q1 = q - (q.AN / alfa_N) N,   Q = q1 - (q1.AO / alfa_O) O.
+// Since q1.AO = q.AO - g_N (N.AO), both coefficients follow from the two dots q.AN and
+// q.AO of one pass plus the cross term N.AO, which was reduced when N was created.  The
+// new direction Q is then formed on the fly (neighbour values recomputed from q, N, O)
+// by the kernel that applies A to it.
 // ---------------------------------------------------------------------------
-// coef = (q . Ap_j) / alfa_j; stored in st->proj.
-__global__ __launch_bounds__(kThreads) void npsd_proj_dot_kernel(const float* __restrict__ q,
-                                                                 const float* __restrict__ Ap,
-                                                                 const double* __restrict__ alfa_j,
-                                                                 long n, SolveState* st,
-                                                                 double* partials,
-                                                                 unsigned* counter) {
-  if (st->done) return;
-  double red[1] = {0.0};
-  for (long i = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; i < n;
-       i += (long)gridDim.x * kThreads)
-    red[0] += (double)q[i] * (double)Ap[i];
-  block_partials<1>(red, partials);
-}
-
-// q -= proj * p_j
-__global__ __launch_bounds__(kThreads) void npsd_axpy_kernel(float* __restrict__ q,
-                                                             const float* __restrict__ pj, long n,
-                                                             const SolveState* st) {
-  if (st->done) return;
-  const float c = (float)st->proj;
-  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
-    q[i] = q[i] - c * pj[i];
-}
-
-// Aq = A q (flat), alfa = q.Aq, beta = q.r / alfa; beta goes to st->alpha so the
-// residual kernel applies x += beta q.
-template <int VZ>
-__global__ __launch_bounds__(kThreads) void npsd_spmv_flat_kernel(FlatArgs a, double* alfa_out) {
-  SolveState* st = a.st;
+// [q.AN, q.AO] (AO may be null).
+__global__ __launch_bounds__(kThreads) void npsd_proj_kernel(const float* __restrict__ q,
+                                                             const float* __restrict__ aN,
+                                                             const float* __restrict__ aO, long n,
+                                                             SolveState* st, double* partials) {
   if (st->done) return;
   double red[2] = {0.0, 0.0};
+  const long nr = n / kDotRun;
+  for (long i = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; i < nr;
+       i += (long)gridDim.x * kThreads) {
+    float qv[kDotRun], nv[kDotRun], ov[kDotRun];
+    load_vec<kDotRun>(q + kDotRun * i, qv);
+    load_vec<kDotRun>(aN + kDotRun * i, nv);
+    if (aO) load_vec<kDotRun>(aO + kDotRun * i, ov);
+#pragma unroll
+    for (int t = 0; t < kDotRun; ++t) red[0] += (double)qv[t] * (double)nv[t];
+    if (aO) {
+#pragma unroll
+      for (int t = 0; t < kDotRun; ++t) red[1] += (double)qv[t] * (double)ov[t];
+    }
+  }
+  if (blockIdx.x == 0) {
+    for (long e = nr * kDotRun + threadIdx.x; e < n; e += kThreads) {
+      red[0] += (double)q[e] * (double)aN[e];
+      if (aO) red[1] += (double)q[e] * (double)aO[e];
+    }
+  }
+  block_partials<2>(red, partials);
+}
+
+// Q = q - g_N N - g_O O, in FP64 with explicit FMAs (the same rounding wherever a value
+// is formed, so the stored Q and the neighbour values used for A Q agree bit for bit).
+struct DirIn {
+  const float* q;
+  const float* pN;  // null when no direction is kept
+  const float* pO;  // null when fewer than two are kept
+  double gN, gO;
+};
+
+__device__ __forceinline__ float dir_at(const DirIn& d, long v) {
+  double u = (double)d.q[v];
+  if (d.pN) u = fma(-d.gN, (double)d.pN[v], u);
+  if (d.pO) u = fma(-d.gO, (double)d.pO[v], u);
+  return (float)u;
+}
+
+template <int VZ>
+__device__ __forceinline__ void dir_vec(const DirIn& d, long v0, float (&out)[VZ]) {
+  float qv[VZ], nv[VZ], ov[VZ];
+  load_vec<VZ>(d.q + v0, qv);
+  if (d.pN) load_vec<VZ>(d.pN + v0, nv);
+  if (d.pO) load_vec<VZ>(d.pO + v0, ov);
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) {
+    double u = (double)qv[t];
+    if (d.pN) u = fma(-d.gN, (double)nv[t], u);
+    if (d.pO) u = fma(-d.gO, (double)ov[t], u);
+    out[t] = (float)u;
+  }
+}
+
+// Q on the 7-point neighbourhood of a run (layout of load_star).
+template <int VZ>
+__device__ __forceinline__ void load_dir_star(const DirIn& d, const Run<VZ>& r, int n1, int n2,
+                                              int n3, float (&c)[VZ + 2], float (&s)[4][VZ]) {
+  const long s1 = (long)n2 * n3;
+  float cv[VZ];
+  dir_vec<VZ>(d, r.v0, cv);
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) c[t + 1] = cv[t];
+  c[0] = r.k0 > 0 ? dir_at(d, r.v0 - 1) : 0.f;
+  c[VZ + 1] = r.k0 + VZ < n3 ? dir_at(d, r.v0 + VZ) : 0.f;
+  const long offs[4] = {-s1, s1, -(long)n3, (long)n3};
+  const bool ok[4] = {r.i > 0, r.i < n1 - 1, r.j > 0, r.j < n2 - 1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (ok[q]) {
+      dir_vec<VZ>(d, r.v0 + offs[q], s[q]);
+    } else {
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) s[q][t] = 0.f;
+    }
+  }
+}
+
+__device__ __forceinline__ DirIn dir_in(const float* q, const float* pN, const float* pO,
+                                        const SolveState* st) {
+  DirIn d;
+  d.q = q;
+  d.pN = pN;
+  d.pO = pO;
+  d.gN = pN ? st->proj[0] : 0.0;
+  d.gO = pO ? st->proj[1] : 0.0;
+  return d;
+}
+
+// Q -> a.p_new, A Q -> a.w; reduces [Q.AQ, Q.r, Q.AN] (AN = a.p, may be null).
+// a.z = q, a.p_old = O; N comes in as pN.
+template <int VZ>
+__global__ __launch_bounds__(kThreads) void npsd_dir_flat_kernel(FlatArgs a, const float* pN,
+                                                                 const float* aN) {
+  SolveState* st = a.st;
+  if (st->done) return;
+  const DirIn d = dir_in(a.z, pN, pN ? a.p_old : nullptr, st);
+  double red[3] = {0.0, 0.0, 0.0};
   const long nch = (long)a.n1 * a.n2 * (a.n3 / VZ);
   for (long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; cid < nch;
        cid += (long)gridDim.x * kThreads) {
     Run<VZ> r;
     run_at<VZ>(cid, a.n1, a.n2, a.n3, r);
     float c[VZ + 2], s[4][VZ];
-    load_star<VZ, float>(a.z, nullptr, 0.f, r, a.n1, a.n2, a.n3, c, s);
-    float code[VZ], rv[VZ], wv[VZ];
+    load_dir_star<VZ>(d, r, a.n1, a.n2, a.n3, c, s);
+    float code[VZ], rv[VZ], wv[VZ], qn[VZ], av[VZ];
     load_vec<VZ>(a.code + r.v0, code);
     load_vec<VZ>(a.r + r.v0, rv);
+    if (aN) load_vec<VZ>(aN + r.v0, av);
 #pragma unroll
     for (int t = 0; t < VZ; ++t) {
       const int cd = (int)code[t];
+      qn[t] = cd ? c[t + 1] : 0.f;
       wv[t] = cd ? lap_at<VZ, float>(c, s, t, cd) : 0.f;
-      red[0] += (double)c[t + 1] * (double)wv[t];
-      red[1] += (double)c[t + 1] * (double)rv[t];
+      red[0] += (double)qn[t] * (double)wv[t];
+      red[1] += (double)qn[t] * (double)rv[t];
+      if (aN) red[2] += (double)qn[t] * (double)av[t];
     }
+    store_vec<VZ>(a.p_new + r.v0, qn);
     store_vec<VZ>(a.w + r.v0, wv);
   }
-  block_partials<2>(red, a.partials);
+  block_partials<3>(red, a.partials);
 }
 
-__global__ __launch_bounds__(kThreads) void npsd_spmv_csr_kernel(CsrArgs a, double* alfa_out) {
+__global__ __launch_bounds__(kThreads) void npsd_dir_csr_kernel(CsrArgs a, const float* pN,
+                                                                const float* aN) {
   SolveState* st = a.st;
   if (st->done) return;
-  double red[2] = {0.0, 0.0};
+  const DirIn d = dir_in(a.z, pN, pN ? a.p_old : nullptr, st);
+  double red[3] = {0.0, 0.0, 0.0};
   for (long row = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; row < a.n;
        row += (long)gridDim.x * kThreads) {
     float w = 0.f;
-    for (int e = a.rowptr[row]; e < a.rowptr[row + 1]; ++e) w += a.val[e] * a.z[a.col[e]];
+    for (int e = a.rowptr[row]; e < a.rowptr[row + 1]; ++e) w += a.val[e] * dir_at(d, a.col[e]);
+    const float qn = dir_at(d, row);
+    a.p_new[row] = qn;
     a.w[row] = w;
-    red[0] += (double)a.z[row] * (double)w;
-    red[1] += (double)a.z[row] * (double)a.r[row];
+    red[0] += (double)qn * (double)w;
+    red[1] += (double)qn * (double)a.r[row];
+    if (aN) red[2] += (double)qn * (double)aN[row];
   }
-  block_partials<2>(red, a.partials);
+  block_partials<3>(red, a.partials);
 }
-
 
 // ---------------------------------------------------------------------------
 // Finalisation of the two-kernel reductions: one block sums the partials in block order
@@ -414,12 +501,14 @@ __global__ __launch_bounds__(kThreads) void npsd_spmv_csr_kernel(CsrArgs a, doub
 // ---------------------------------------------------------------------------
 enum FinOp : int { kFinResidInit = 0, kFinResid, kFinPW, kFinRZ, kFinDot, kFinProj, kFinNpsd };
 
+// aux/aux2: the output of kFinDot; alfa of N and O for kFinProj; alfa of the new
+// direction for kFinNpsd.
 __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
                                                                const double* __restrict__ partials,
                                                                unsigned nb, SolveState* st,
-                                                               double* aux) {
+                                                               double* aux, const double* aux2) {
   if (op != kFinResidInit && op != kFinDot && st->done) return;  // uniform over the block
-  double t[2] = {0.0, 0.0};
+  double t[3] = {0.0, 0.0, 0.0};
   for (int r = 0; r < nr; ++r) t[r] = sum_partials(partials + (size_t)r * nb, nb);
   if (threadIdx.x != 0) return;
   switch (op) {
@@ -457,19 +546,23 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
     case kFinDot:
       aux[0] = t[0];
       break;
-    case kFinProj:
-      st->proj = t[0] / aux[0];
+    case kFinProj: {
+      const double gN = t[0] / aux[0];
+      st->proj[0] = gN;
+      st->proj[1] = aux2 ? (t[1] - gN * st->cross) / aux2[0] : 0.0;
       break;
+    }
     case kFinNpsd:
       aux[0] = t[0];
       st->alpha = t[1] / t[0];
+      st->cross = t[2];
       break;
   }
 }
 
 static void finalize(int op, int nr, double* partials, unsigned nb, SolveState* st, double* aux,
-                     hipStream_t s) {
-  finalize_kernel<<<1, kFinThreads, 0, s>>>(op, nr, partials, nb, st, aux);
+                     hipStream_t s, const double* aux2 = nullptr) {
+  finalize_kernel<<<1, kFinThreads, 0, s>>>(op, nr, partials, nb, st, aux, aux2);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,10 +612,11 @@ int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st) {
   return check_launch("p_spmv_flat");
 }
 
-int launch_npsd_spmv_flat(const FlatArgs& a, double* alfa_out, hipStream_t st) {
-  MLPCG_FLAT_DISPATCH(npsd_spmv_flat_kernel, 0, a, alfa_out);
-  finalize(kFinNpsd, 2, a.partials, flat_blocks(a), a.st, alfa_out, st);
-  return check_launch("npsd_spmv_flat");
+int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, const float* aN, double* alfa_out,
+                         hipStream_t st) {
+  MLPCG_FLAT_DISPATCH(npsd_dir_flat_kernel, 0, a, pN, aN);
+  finalize(kFinNpsd, 3, a.partials, flat_blocks(a), a.st, alfa_out, st);
+  return check_launch("npsd_dir_flat");
 }
 
 int launch_rz_dot(const float* r, const float* z, const float* r_old, long n, SolveState* s,
@@ -559,24 +653,20 @@ int launch_resid_csr(const CsrArgs& a, int init, hipStream_t st) {
   return check_launch("resid_csr");
 }
 
-int launch_npsd_spmv_csr(const CsrArgs& a, double* alfa_out, hipStream_t st) {
+int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, const float* aN, double* alfa_out,
+                        hipStream_t st) {
   const unsigned nb = red_blocks(a.n);
-  npsd_spmv_csr_kernel<<<nb, kThreads, 0, st>>>(a, alfa_out);
-  finalize(kFinNpsd, 2, a.partials, nb, a.st, alfa_out, st);
-  return check_launch("npsd_spmv_csr");
+  npsd_dir_csr_kernel<<<nb, kThreads, 0, st>>>(a, pN, aN);
+  finalize(kFinNpsd, 3, a.partials, nb, a.st, alfa_out, st);
+  return check_launch("npsd_dir_csr");
 }
 
-int launch_npsd_proj(const float* q, const float* Ap, const double* alfa_j, long n, SolveState* s,
-                     double* partials, unsigned* counter, hipStream_t st) {
-  const unsigned nb = red_blocks(n);
-  npsd_proj_dot_kernel<<<nb, kThreads, 0, st>>>(q, Ap, alfa_j, n, s, partials, counter);
-  finalize(kFinProj, 1, partials, nb, s, const_cast<double*>(alfa_j), st);
+int launch_npsd_proj(const float* q, const float* aN, const float* aO, const double* alfaN,
+                     const double* alfaO, long n, SolveState* s, double* partials, hipStream_t st) {
+  const unsigned nb = red_blocks(n / kDotRun + 1);
+  npsd_proj_kernel<<<nb, kThreads, 0, st>>>(q, aN, aO, n, s, partials);
+  finalize(kFinProj, 2, partials, nb, s, const_cast<double*>(alfaN), st, alfaO);
   return check_launch("npsd_proj");
-}
-
-int launch_npsd_axpy(float* q, const float* pj, long n, const SolveState* s, hipStream_t st) {
-  npsd_axpy_kernel<<<red_blocks(n), kThreads, 0, st>>>(q, pj, n, s);
-  return check_launch("npsd_axpy");
 }
 
 int launch_spmv_csr(const int* rowptr, const int* col, const float* val, const float* x, float* y,

@@ -10,12 +10,13 @@ constexpr int8_t kFlagFluid = 2;  // lib/read_data.py:13-16: BOUNDARY 0, SOLID 1
 enum Method : int { kNpcg = 0, kNpcgFlex = 1, kNpsd = 2 };
 
 // Solver scalars, resident in device memory for the whole solve.  Updated only by the
-// last block of each reduction kernel.
+// one-block finalize kernels of the reductions.
 struct SolveState {
   double rho, rho_old, pw, alpha, beta;
   double rr, norm_b, norm, scale;  // scale = 1 / ||r|| (input normalisation of P)
   double tol, atol;
-  double proj;                     // npsd projection coefficient
+  double proj[2];                  // npsd projection coefficients (newest, older direction)
+  double cross;                    // npsd: newest direction . A (previous direction)
   double alfa[2];                  // npsd: q.Aq of the last two directions
   double* history;                 // optional device array [max_it + 1] of ||r||/||b||
   int iter, max_it, done, method;
@@ -61,7 +62,8 @@ size_t reduction_workspace_bytes();
 int launch_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3, hipStream_t st);
 int launch_resid_flat(const FlatArgs& a, int init, hipStream_t st);
 int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st);
-int launch_npsd_spmv_flat(const FlatArgs& a, double* alfa_out, hipStream_t st);
+int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, const float* aN, double* alfa_out,
+                         hipStream_t st);
 int launch_rz_dot(const float* r, const float* z, const float* r_old, long n, SolveState* s,
                   double* partials, unsigned* counter, hipStream_t st);
 int launch_scatter(const int* idx, const float* src, float* grid, long n, const int* done,
@@ -70,10 +72,10 @@ int launch_gather(const int* idx, const float* grid, float* dst, long n, const i
                   hipStream_t st);
 int launch_p_spmv_csr(const CsrArgs& a, hipStream_t st);
 int launch_resid_csr(const CsrArgs& a, int init, hipStream_t st);
-int launch_npsd_spmv_csr(const CsrArgs& a, double* alfa_out, hipStream_t st);
-int launch_npsd_proj(const float* q, const float* Ap, const double* alfa_j, long n, SolveState* s,
-                     double* partials, unsigned* counter, hipStream_t st);
-int launch_npsd_axpy(float* q, const float* pj, long n, const SolveState* s, hipStream_t st);
+int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, const float* aN, double* alfa_out,
+                        hipStream_t st);
+int launch_npsd_proj(const float* q, const float* aN, const float* aO, const double* alfaN,
+                     const double* alfaO, long n, SolveState* s, double* partials, hipStream_t st);
 int launch_spmv_csr(const int* rowptr, const int* col, const float* val, const float* x, float* y,
                     long n, hipStream_t st);
 int launch_spmv_csr_f64(const int* rowptr, const int* col, const double* val, const double* x,

@@ -23,6 +23,10 @@ static size_t dsize(int dt) {
   }
 }
 
+static int g_fused_apply = 0;
+void set_fused_apply(int enable) { g_fused_apply = enable ? 1 : 0; }
+int fused_apply_enabled() { return g_fused_apply; }
+
 Precond::~Precond() {
   for (void* p : allocs) (void)hipFree(p);
 }
@@ -85,29 +89,39 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
   }
   // stencils: (weight index, level, transposed)
   if (kind == kSmallSM) {
-    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0});          // pre
-    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 0});      // post
-    P->blocks.push_back({2 * n, n, 0});                                  // l
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0, 0});       // pre
+    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 0, 0});   // post
+    P->blocks.push_back({2 * n, n, 0, 0});                               // l
     P->nw = 2 * n + 1;
   } else if (kind == kSPD) {
-    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0});          // l0
-    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 0});      // l1
-    P->blocks.push_back({2 * n, n, 0});                                  // l
-    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 1});      // l1_t
-    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 1});          // l0_t
-    P->blocks.push_back({2 * n, n, 1});                                  // l_t
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0, 0});       // l0
+    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 0, 0});   // l1
+    P->blocks.push_back({2 * n, n, 0, 0});                               // l
+    for (int i = 0; i < n; ++i) P->blocks.push_back({n + i, i, 1, 0});   // l1_t
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 1, 0});       // l0_t
+    P->blocks.push_back({2 * n, n, 1, 0});                               // l_t
     P->nw = 2 * n + 1;
   } else {
-    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0});          // l0
-    P->blocks.push_back({n, n, 0});                                      // l
-    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 1});          // l0_t
-    P->blocks.push_back({n, n, 1});                                      // l_t
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 0, 0});       // l0
+    P->blocks.push_back({n, n, 0, 0});                                   // l
+    for (int i = 0; i < n; ++i) P->blocks.push_back({i, i, 1, 0});       // l0_t
+    P->blocks.push_back({n, n, 1, 0});                                   // l_t
     P->nw = n + 1;
   }
-  for (const Precond::Blk& b : P->blocks) {
+  const bool half = (dtype == kBF16 || dtype == kF16);
+  for (Precond::Blk& b : P->blocks) {
+    const int* d = P->dims[b.level];
     void* K = nullptr;
-    TRY(palloc(P, &K, (size_t)27 * P->V[b.level] * es));
+    void* wp = nullptr;
+    b.fused = (half && !b.trans && g_fused_apply && fused_eligible(C, 1, d[0], d[1], d[2])) ? 1 : 0;
+    if (b.fused) {
+      TRY(palloc(P, &wp, fused_weights_bytes(C)));
+      if (!P->imgw[b.level]) TRY(palloc(P, &P->imgw[b.level], (size_t)C * P->V[b.level] * es));
+    } else {
+      TRY(palloc(P, &K, (size_t)27 * P->V[b.level] * es));
+    }
     P->K.push_back(K);
+    P->wpack.push_back(wp);
   }
   TRY(palloc(P, &P->coef, (size_t)4 * n * as));
   TRY(palloc(P, &P->lin_ws, mean_patch_workspace(P->acc, C, n1, n2, n3)));
@@ -136,11 +150,16 @@ int Precond::update(const void* image, const void* const* weights, const void* c
     TRY(launch_avgpool2(acc, img[l - 1], img[l], C, dims[l - 1][0], dims[l - 1][1], dims[l - 1][2],
                         nullptr, st));
   if (mask) TRY(launch_nonzero_mask(acc, (const char*)img[0] + (size_t)V[0] * as, maskimg, V[0], st));
+  for (int l = 0; l <= n; ++l)
+    if (imgw[l]) TRY(launch_cast(acc, dtype, img[l], imgw[l], (long)C * V[l], st));
   for (size_t k = 0; k < blocks.size(); ++k) {
     const Blk& b = blocks[k];
     const int* d = dims[b.level];
-    TRY(launch_stencil_weights2(acc, dtype, img[b.level], weights[b.w], biases[b.w], K[k], C, d[0], d[1],
-                                d[2], b.trans != 0, st));
+    if (b.fused)
+      TRY(launch_fused_pack(dtype, weights[b.w], biases[b.w], C, wpack[k], st));
+    else
+      TRY(launch_stencil_weights2(acc, dtype, img[b.level], weights[b.w], biases[b.w], K[k], C, d[0],
+                                  d[1], d[2], b.trans != 0, st));
   }
   // SMLinear coefficients c0[i], c1[i] from img[i] (plus their squares for kSmallSPD)
   for (int which = 0; which < 2; ++which) {
@@ -166,6 +185,25 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
   auto apply_blk = [&](int k, int level, const void* x, int tx, void* y, int ty, const void* prev,
                        const void* c0, const void* c1, const uint8_t* ma, const uint8_t* mb,
                        const double* sc) {
+    if (blocks[k].fused) {
+      FusedArgs f;
+      f.img = imgw[level];
+      f.apack = wpack[k];
+      f.binit = (const float*)((const char*)wpack[k] + fused_weights_bytes(C) - 2 * 64 * 4 * sizeof(float));
+      f.x = x;
+      f.y = y;
+      f.prev = prev;
+      f.c0 = c0;
+      f.c1 = c1;
+      f.mask_a = ma;
+      f.mask_b = mb;
+      f.in_scale = sc;
+      f.done = done;
+      f.n1 = dims[level][0];
+      f.n2 = dims[level][1];
+      f.n3 = dims[level][2];
+      return launch_sm_fused(dtype, tx, ty, C, f, st);
+    }
     ApplyArgs a;
     a.K = K[k];
     a.x = x;

@@ -14,7 +14,9 @@ enum ModelKind : int { kSmallSM = 0, kSPD = 1, kSmallSPD = 2 };
 
 // Everything that depends only on the image is computed once at creation: the image
 // pyramid (accumulate precision), the stencil weights K of every block (storage
-// precision, [27][V_level]) and the SMLinear coefficients c0/c1 (device scalars).
+// precision, [27][V_level]) and the SMLinear coefficients c0/c1 (device scalars).  For
+// 16-bit plans, blocks on grids the fused kernel tiles keep no K: they hold the weights
+// packed for the matrix cores and a 16-bit copy of their level's image instead.
 // This is the caching that SmallSMModelDn3D.eval_forward does for imgs/c0/c1 through
 // its list arguments (sm_model_3d.py:180-195), extended to K.
 struct Precond {
@@ -33,8 +35,11 @@ struct Precond {
   uint8_t* maskimg = nullptr;       // level-0 model mask: img[1] != 0
   struct Blk {
     int w, level, trans;            // weight index, level, transposed stencil
+    int fused;                      // 1: K recomputed on the matrix cores (no K stored)
   };
-  std::vector<Blk> blocks;          // one per entry of K
+  std::vector<Blk> blocks;          // one per entry of K (nullptr for fused blocks)
+  std::vector<void*> wpack;         // fused blocks: packed MFMA weights + bias init
+  void* imgw[kMaxLevels + 1] = {};  // 16-bit image of levels with fused blocks
   int nw = 0;                       // number of block weights before c0/c1
   void* lin_ws = nullptr;           // SMLinear reduction workspace
   void* ybar = nullptr;             // SMLinear mean patch scratch
@@ -47,6 +52,11 @@ struct Precond {
             const int* done, hipStream_t st);
   ~Precond();
 };
+
+// Plans created while this is set use the fused matrix-core apply for eligible 16-bit
+// blocks (process-wide; default off).
+void set_fused_apply(int enable);
+int fused_apply_enabled();
 
 int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3, int dtype,
                    int nearest, int mask, int swap, const void* image, const void* const* weights,
@@ -73,8 +83,8 @@ struct Solver {
   float* p[2] = {};
   float* z = nullptr;
   float* w = nullptr;
-  float* q_hist[2] = {};   // npsd: previous directions
-  float* aq_hist[2] = {};  // npsd: their A-products
+  float* q_hist[3] = {};   // npsd: directions, ring of 3 (two kept + the one being formed)
+  float* aq_hist[2] = {};  // npsd: A-products of the two kept directions
   float* grid_in = nullptr;
   float* grid_out = nullptr;
   SolveState* st = nullptr;
@@ -86,7 +96,8 @@ struct Solver {
   hipStream_t side = nullptr;
   hipEvent_t ev_iter[4] = {}, ev_stat[4] = {};
   std::vector<void*> allocs;
-  // HIP-graph replay of the iteration (two graphs: even / odd buffer parity)
+  // HIP-graph replay of the iteration (two graphs: even / odd buffer parity; npcg and
+  // npcg_flex only, the npsd direction ring has period 6)
   int use_graphs = 0;
   hipStream_t work = nullptr;
   hipEvent_t ev_in = nullptr;

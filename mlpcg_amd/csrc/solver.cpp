@@ -46,9 +46,10 @@ static int solver_common(Solver* S, hipStream_t st) {
     if (salloc(S, (void**)&S->x[i], vb, true, st)) return -1;
     if (salloc(S, (void**)&S->r[i], vb, true, st)) return -1;
     if (salloc(S, (void**)&S->p[i], vb, true, st)) return -1;
-    if (salloc(S, (void**)&S->q_hist[i], vb, true, st)) return -1;
     if (salloc(S, (void**)&S->aq_hist[i], vb, true, st)) return -1;
   }
+  for (int i = 0; i < 3; ++i)
+    if (salloc(S, (void**)&S->q_hist[i], vb, true, st)) return -1;
   if (salloc(S, (void**)&S->z, vb, true, st)) return -1;
   if (salloc(S, (void**)&S->w, vb, true, st)) return -1;
   if (salloc(S, (void**)&S->st, sizeof(SolveState), true, st)) return -1;
@@ -246,36 +247,42 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
       }
       return 0;
     }
-    // npsd: q = P(r); q -= (q.Ap_j / alfa_j) p_j for the kept directions, newest first
+    // npsd: q = P(r), projected against the kept directions N (newest) and O; direction
+    // d lives in q_hist[d % 3], its A-product and alfa in slot d % 2.
     float* q = z;
     TRY2(precondition(rk, q, s));
     const int kept = cnt < 2 ? cnt : 2;
-    for (int jj = kept - 1; jj >= 0; --jj) {
-      const int slot = (cnt - kept + jj) & 1;  // ring position of direction jj
-      TRY2(launch_npsd_proj(q, aq_hist[slot], &this->st->alfa[slot], n, this->st, partials, counter, s));
-      TRY2(launch_npsd_axpy(q, q_hist[slot], n, this->st, s));
-    }
-    // the new direction replaces the oldest slot
-    const int slot_new = cnt & 1;
-    MLPCG_HIP(hipMemcpyAsync(q_hist[slot_new], q, vb, hipMemcpyDeviceToDevice, s));
+    const float* pN = kept >= 1 ? q_hist[(cnt - 1) % 3] : nullptr;
+    const float* pO = kept >= 2 ? q_hist[(cnt - 2) % 3] : nullptr;
+    const float* aN = kept >= 1 ? aq_hist[(cnt - 1) & 1] : nullptr;
+    if (kept >= 1)
+      TRY2(launch_npsd_proj(q, aN, kept >= 2 ? aq_hist[cnt & 1] : nullptr,
+                            &this->st->alfa[(cnt - 1) & 1],
+                            kept >= 2 ? &this->st->alfa[cnt & 1] : nullptr, n, this->st, partials,
+                            s));
+    float* q_new = q_hist[cnt % 3];
     if (layout == 0) {
-      fa.z = q_hist[slot_new];
+      fa.z = q;
+      fa.p_old = pO;
+      fa.p_new = q_new;
       fa.r = const_cast<float*>(rk);
-      fa.w = aq_hist[slot_new];
-      TRY2(launch_npsd_spmv_flat(fa, &this->st->alfa[slot_new], s));
+      fa.w = aq_hist[cnt & 1];
+      TRY2(launch_npsd_dir_flat(fa, pN, aN, &this->st->alfa[cnt & 1], s));
       fa.x_old = x[cur];
       fa.x_new = x[nxt];
-      fa.p = q_hist[slot_new];
+      fa.p = q_new;
       fa.r = r[nxt];
       TRY2(launch_resid_flat(fa, 0, s));
     } else {
-      ca.z = q_hist[slot_new];
+      ca.z = q;
+      ca.p_old = pO;
+      ca.p_new = q_new;
       ca.r = const_cast<float*>(rk);
-      ca.w = aq_hist[slot_new];
-      TRY2(launch_npsd_spmv_csr(ca, &this->st->alfa[slot_new], s));
+      ca.w = aq_hist[cnt & 1];
+      TRY2(launch_npsd_dir_csr(ca, pN, aN, &this->st->alfa[cnt & 1], s));
       ca.x_old = x[cur];
       ca.x_new = x[nxt];
-      ca.p = q_hist[slot_new];
+      ca.p = q_new;
       ca.r = r[nxt];
       TRY2(launch_resid_csr(ca, 0, s));
     }
@@ -286,9 +293,9 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
   int k = 0;
   int npsd_count = 0;  // directions created (npsd)
   for (k = 1; k <= max_it; ++k) {
-    // iterations k and k+2 launch the same kernels with the same arguments (buffer
-    // parity k & 1; for npsd the direction ring is periodic once two directions exist)
-    const bool replay = use_graphs && (method != kNpsd || npsd_count >= 2);
+    // npcg / npcg_flex: iterations k and k+2 launch the same kernels with the same
+    // arguments (buffer parity k & 1), so two captured graphs cover the solve
+    const bool replay = use_graphs && method != kNpsd;
     if (replay) {
       const int par = k & 1;
       if (!graph_exec[par]) {

@@ -184,6 +184,20 @@ __global__ __launch_bounds__(kThreads) void sm_apply_kernel(ApplyArgs a) {
   const int b0 = blockIdx.y * BB;
   const TK* __restrict__ K = (const TK*)a.K;
   const TX* __restrict__ x = (const TX*)a.x;
+  // epilogue operands are loaded first so their latency overlaps the stencil loads (on
+  // the small coarse-level launches the kernel is a chain of memory round trips)
+  const A scale = a.in_scale ? (A)(*a.in_scale) : A(1);
+  const A c0 = a.prev ? *(const A*)a.c0 : A(0);
+  const A c1 = a.prev ? *(const A*)a.c1 : A(0);
+  uint8_t keep_a[VZ], keep_b[VZ];
+  if (a.mask_a) load_vec<VZ>(a.mask_a + c.v0, keep_a);
+  if (a.mask_b) load_vec<VZ>(a.mask_b + c.v0, keep_b);
+  TP pv[BB][VZ];
+  if (a.prev) {
+#pragma unroll
+    for (int bb = 0; bb < BB; ++bb)
+      if (!(BB > 1 && b0 + bb >= a.B)) load_vec<VZ>((const TP*)a.prev + (long)(b0 + bb) * V + c.v0, pv[bb]);
+  }
   A acc[BB][VZ];
 #pragma unroll
   for (int bb = 0; bb < BB; ++bb)
@@ -212,12 +226,6 @@ __global__ __launch_bounds__(kThreads) void sm_apply_kernel(ApplyArgs a) {
   }
 
   // epilogue
-  const A scale = a.in_scale ? (A)(*a.in_scale) : A(1);
-  A keep_a[VZ], keep_b[VZ];
-  if (a.mask_a) load_vec<VZ>(a.mask_a + c.v0, keep_a);
-  if (a.mask_b) load_vec<VZ>(a.mask_b + c.v0, keep_b);
-  const A c0 = a.prev ? *(const A*)a.c0 : A(0);
-  const A c1 = a.prev ? *(const A*)a.c1 : A(0);
 #pragma unroll
   for (int bb = 0; bb < BB; ++bb) {
     if (BB > 1 && b0 + bb >= a.B) break;
@@ -226,19 +234,17 @@ __global__ __launch_bounds__(kThreads) void sm_apply_kernel(ApplyArgs a) {
 #pragma unroll
     for (int t = 0; t < VZ; ++t) {
       A val = acc[bb][t] * scale;
-      if (a.mask_a && keep_a[t] == A(0)) val = A(0);
+      if (a.mask_a && keep_a[t] == 0) val = A(0);
       out[t] = val;
     }
     if (a.prev) {
-      A pv[VZ];
-      load_vec<VZ>((const TP*)a.prev + off, pv);
 #pragma unroll
-      for (int t = 0; t < VZ; ++t) out[t] = c0 * pv[t] + c1 * out[t];
+      for (int t = 0; t < VZ; ++t) out[t] = c0 * (A)pv[bb][t] + c1 * out[t];
     }
     if (a.mask_b) {
 #pragma unroll
       for (int t = 0; t < VZ; ++t)
-        if (keep_b[t] == A(0)) out[t] = A(0);
+        if (keep_b[t] == 0) out[t] = A(0);
     }
     store_vec<VZ>((TY*)a.y + off, out);
   }
@@ -323,6 +329,11 @@ __global__ __launch_bounds__(kThreads) void upsample2_kernel(UpArgs a) {
   const TI* r10 = in + ((long)i1 * m2 + j0) * m3;
   const TI* r11 = in + ((long)i1 * m2 + j1) * m3;
   const A hi0 = A(1) - (A)li, hi1 = (A)li, hj0 = A(1) - (A)lj, hj1 = (A)lj;
+  const long off = b * Vf + c.v0;
+  // combine operands first (latency overlaps the coarse loads)
+  const A c0 = a.prev ? *(const A*)a.c0 : A(0), c1 = a.prev ? *(const A*)a.c1 : A(0);
+  TP pv[VZ];
+  if (a.prev) load_vec<VZ>((const TP*)a.prev + off, pv);
   A out[VZ];
   if constexpr (VZ >= 2) {
     // coarse z segment [k0/2 - 1, k0/2 + VZ/2] of the four rows, indices clamped to the
@@ -364,13 +375,9 @@ __global__ __launch_bounds__(kThreads) void upsample2_kernel(UpArgs a) {
       out[t] = (hi0 * v0 + hi1 * v1) * (A)a.scale;
     }
   }
-  const long off = b * Vf + c.v0;
   if (a.prev) {
-    const A c0 = *(const A*)a.c0, c1 = *(const A*)a.c1;
-    A pv[VZ];
-    load_vec<VZ>((const TP*)a.prev + off, pv);
 #pragma unroll
-    for (int t = 0; t < VZ; ++t) out[t] = c0 * pv[t] + c1 * out[t];
+    for (int t = 0; t < VZ; ++t) out[t] = c0 * (A)pv[t] + c1 * out[t];
   }
   store_vec<VZ>((TO*)a.out + off, out);
 }
@@ -428,6 +435,15 @@ static int pick_vz(int n3, int maxvz) {
   for (int vz = maxvz; vz > 1; vz >>= 1)
     if (n3 % vz == 0) return vz;
   return 1;
+}
+
+// Launches over fewer voxels than this skip the solver's done flag: their run time is a
+// chain of memory round trips, and the flag load would add one; the work a speculative
+// iteration then does on them is a few KB.
+constexpr long kDoneCheckMinVoxels = 1L << 20;
+
+static inline const int* done_for(const int* done, long voxels) {
+  return voxels >= kDoneCheckMinVoxels ? done : nullptr;
 }
 
 static inline dim3 grid1(long items, int y = 1) {
@@ -498,9 +514,13 @@ static void launch_apply_t(const ApplyArgs& a, int vz, hipStream_t st) {
 #undef MLPCG_AP
 }
 
-int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& a, hipStream_t st) {
-  MLPCG_REQUIRE(a.B >= 1 && a.n1 >= 1 && a.n2 >= 1 && a.n3 >= 1, "sm_apply: bad sizes");
-  const int vz = pick_vz(a.n3, tk == kF64 ? 4 : 8);
+int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& args, hipStream_t st) {
+  MLPCG_REQUIRE(args.B >= 1 && args.n1 >= 1 && args.n2 >= 1 && args.n3 >= 1, "sm_apply: bad sizes");
+  ApplyArgs a = args;
+  a.done = done_for(a.done, (long)a.n1 * a.n2 * a.n3);
+  // 16-byte stencil loads per tap and thread: VZ = 8 (16-bit), 4 (FP32), 2 (FP64); wider
+  // runs hoist more loads than the register file holds at two waves per SIMD
+  const int vz = pick_vz(a.n3, tk == kF64 ? 2 : (tk == kF32 ? 4 : 8));
   if (tk == kF64) {
     MLPCG_REQUIRE(tx == kF64 && ty == kF64, "sm_apply: f64 stencil needs f64 vectors");
     launch_apply_t<double, double, double, double>(a, vz, st);
@@ -543,6 +563,7 @@ int launch_avgpool2(int dtype, const void* in, void* out, int B, int n1, int n2,
   MLPCG_REQUIRE(n1 % 2 == 0 && n2 % 2 == 0 && n3 % 2 == 0,
                 "avgpool2: grid (%d,%d,%d) must be even in every dimension", n1, n2, n3);
   const int c1 = n1 / 2, c2 = n2 / 2, c3 = n3 / 2;
+  done = done_for(done, (long)n1 * n2 * n3);
   switch (dtype) {
     case kF64: launch_pool_t<double>(in, out, B, c1, c2, c3, done, st); break;
     case kF32: launch_pool_t<float>(in, out, B, c1, c2, c3, done, st); break;
@@ -563,9 +584,11 @@ static void launch_up_t(const UpArgs& a, hipStream_t st) {
   else upsample2_kernel<TI, TO, TO, 1><<<grid1(items, a.B), kThreads, 0, st>>>(a);
 }
 
-int launch_upsample2(int dtype, const UpArgs& a, hipStream_t st) {
-  MLPCG_REQUIRE(a.n1 % 2 == 0 && a.n2 % 2 == 0 && a.n3 % 2 == 0,
-                "upsample2: fine grid (%d,%d,%d) must be even", a.n1, a.n2, a.n3);
+int launch_upsample2(int dtype, const UpArgs& args, hipStream_t st) {
+  MLPCG_REQUIRE(args.n1 % 2 == 0 && args.n2 % 2 == 0 && args.n3 % 2 == 0,
+                "upsample2: fine grid (%d,%d,%d) must be even", args.n1, args.n2, args.n3);
+  UpArgs a = args;
+  a.done = done_for(a.done, (long)a.n1 * a.n2 * a.n3);
   switch (dtype) {
     case kF64: launch_up_t<double, double>(a, st); break;
     case kF32: launch_up_t<float, float>(a, st); break;

@@ -1,0 +1,142 @@
+"""GPU parity of the fused matrix-core apply (K recomputed per voxel, fused.hip) for 16-bit
+plans: against the stored-K plan on exactly representable data (tile indexing, halos,
+taps, channels, masks, epilogue), against the FP64 oracle with trained weights, and in a
+solve."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from mlpcg_amd import scenes as S
+from oracle import model as M
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _plan(m, img, dtype, fused):
+    from mlpcg_amd.precond import set_fused_apply
+    prev = set_fused_apply(fused)
+    try:
+        return m.make_preconditioner(img, dtype)
+    finally:
+        set_fused_apply(prev)
+
+
+def _exact_model(n, C, mask, seed):
+    """Block weights in {-2..2}/4 (C=1) or {-1..1}/4 (C=2) and biases in {-2..2}/4, so every
+    stencil value is a multiple of 1/16 below 16 in magnitude: exact in BF16/FP16 and in
+    the split BF16 weights; c0 = 0.5, c1 = 1 exactly."""
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    g = torch.Generator().manual_seed(seed)
+    m = SmallSMModelDn3D(n, C, "trilinear", mask, False)
+    span = 2 if C == 1 else 1
+    with torch.no_grad():
+        for k, p in m.named_parameters():
+            if k.startswith(("c0", "c1")):
+                p.zero_()
+                if k.endswith("bias"):
+                    p.fill_(0.5 if k.startswith("c0") else 1.0)
+            elif k.endswith("weight"):
+                p.copy_(torch.randint(-span, span + 1, p.shape, generator=g).float() / 4)
+            else:
+                p.copy_(torch.randint(-2, 3, p.shape, generator=g).float() / 4)
+    return m.cuda()
+
+
+def _exact_image(C, shape, seed, n):
+    """Values in {0, 1/4, 1/2, 3/4} ({0, 1} for the mask channel), constant on 2^n cubes so
+    that every pyramid level holds the same values and its stencils stay exact."""
+    g = torch.Generator().manual_seed(seed)
+    f = 2 ** n
+    cs = tuple(d // f for d in shape)
+    img = torch.randint(0, 4, (C, *cs), generator=g).float() / 4
+    if C == 2:
+        img[1] = (torch.rand(cs, generator=g) > 0.3).float()
+    for ax in (1, 2, 3):
+        img = img.repeat_interleave(f, dim=ax)
+    return img.contiguous().cuda()
+
+
+def _nrel(a, b):
+    a, b = a.double(), b.double()
+    return float((a - b).norm() / b.norm())
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("C,mask,n,shape", [
+    (1, False, 1, (16, 16, 128)),   # two fused levels, 2x2x2 and 1x1x1 tiles
+    (1, False, 2, (24, 40, 192)),   # odd tile counts; level 1 (12, 20, 96) stays on stored K
+    (2, True, 1, (16, 8, 64)),      # two channels, model mask
+    (2, False, 2, (32, 16, 128)),
+])
+def test_fused_matches_stored_stencils(dev, dtype, C, mask, n, shape):
+    """With exactly representable weights, images and input, both plans compute the same
+    stencils exactly and differ only by FP32 summation order (plus the rare 16-bit
+    rounding flip that causes): a wrong tap, halo or channel offset shows as O(1)."""
+    m = _exact_model(n, C, mask, seed=3 + C)
+    img = _exact_image(C, shape, seed=5, n=n)
+    g = torch.Generator().manual_seed(7)
+    r = torch.randint(-4, 5, shape, generator=g).float().cuda()
+    pf, ps = _plan(m, img, dtype, True), _plan(m, img, dtype, False)
+    assert pf.nbytes < ps.nbytes  # fused blocks keep no stencils
+    zf, zs = pf.apply(r), ps.apply(r)
+    assert torch.isfinite(zf).all()
+    assert _nrel(zf, zs) < 1e-5
+    sc = torch.tensor(0.5, dtype=torch.float64, device=dev)
+    om = (img[0] > 0.3).to(torch.uint8)
+    zf2, zs2 = pf.apply(r, in_scale=sc, out_mask=om), ps.apply(r, in_scale=sc, out_mask=om)
+    assert _nrel(zf2, zs2) < 1e-5
+    assert torch.all(zf2[om == 0] == 0)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.bfloat16, 6e-3), (torch.float16, 1.5e-3)])
+def test_fused_plan_vs_fp64_trained(dev, dtype, tol):
+    """Trained weights on a 64^3 pillars scene: the fused 16-bit plan is as close to the
+    FP64 oracle composition as the stored-K plan of the same type (measured 1.7e-3 BF16 /
+    3.3e-4 FP16 for the stencils alone at 128^3 on the CPU)."""
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    sd = torch.load(os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c1.pt"), weights_only=True)
+    n, N = 5, 64
+    m = SmallSMModelDn3D(n, 1, "trilinear").cuda()
+    m.load_state_dict(sd)
+    sc = S.scene_pillars(N, 0)
+    img = torch.as_tensor(S.image_channels(sc.flags, 1))
+    rng = np.random.default_rng(2)
+    r = torch.as_tensor(rng.standard_normal(N ** 3) * (sc.flags == S.FLUID).ravel()).reshape(N, N, N)
+    P64 = {k: v.double() for k, v in sd.items()}
+    ref = M.model_forward(P64, n, img.double(), r.double().reshape(1, 1, N, N, N)).reshape(N, N, N)
+    pf = _plan(m, img.float().cuda(), dtype, True)
+    ps = _plan(m, img.float().cuda(), dtype, False)
+    ef = _nrel(pf.apply(r.float().cuda()).cpu(), ref)
+    es = _nrel(ps.apply(r.float().cuda()).cpu(), ref)
+    assert ef < tol, (ef, es)
+    assert ef < 2.0 * es + 1e-4, (ef, es)
+
+
+def test_fused_bf16_solve_iterations(dev):
+    """npsd on the held-out 128^3 sphere scene of the shipped weights: the fused BF16 plan
+    converges (FP64-checked residual) within 1.1x the iterations of the FP32 plan."""
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    from mlpcg_amd.solver import FusedPCG
+    sd = torch.load(os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c1.pt"), weights_only=True)
+    N = 128
+    m = SmallSMModelDn3D(5, 1, "trilinear").cuda()
+    m.load_state_dict(sd)
+    sc = S.scene_spheres(N, 7)
+    A = S.build_matrix(sc.flags)
+    b = S.make_rhs(sc.flags, 0)[0]
+    img = torch.as_tensor(S.image_channels(sc.flags, 1)).float().cuda()
+    s = FusedPCG(sc.flags, "flags", device=dev)
+    bv = s.vector(b)
+    its = {}
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        plan = _plan(m, img, dt, True)
+        x, info = s.solve(bv, plan, method="npsd", max_it=300, tol=1e-6)
+        assert info["status"] == 0, (name, info)
+        xs = x.cpu().numpy()[sc.fluid_cells].astype(np.float64)
+        assert np.linalg.norm(b - A @ xs) / np.linalg.norm(b) < 1.05e-6
+        its[name] = info["iters"]
+    assert its["bf16"] <= 1.1 * its["fp32"] + 1, its

@@ -120,6 +120,21 @@ def test_low_precision_plan_solve_runs(dev, dtype):
     assert abs(res - info["rel_res"]) < 1e-6 + 1e-3 * res
 
 
+@pytest.mark.parametrize("layout", ["flags", "csr"])
+def test_npsd_long_history_vs_oracle(dev, layout):
+    """npsd with the identity preconditioner over 40 iterations: both kept directions,
+    the 3-slot direction ring and the cross term N.AO run through many cycles; the
+    residual history follows the FP64 loop of cg_tests.py:189-284."""
+    from mlpcg_amd.solver import FusedPCG
+    sc, A, b = _scene(n=32, seed=9)
+    s = FusedPCG(sc.flags, layout, A=A if layout == "csr" else None, device=dev)
+    it = 40
+    _, info = s.solve(s.vector(b), None, method="npsd", max_it=it, tol=1e-12, history=True)
+    _, k, h_ref = O.npsd(b, A, np.zeros_like(b), lambda r: r.copy(), it, tol=1e-12)
+    assert info["iters"] == k == it
+    assert _hist_close(info["history"], h_ref, it + 1, 1e-3) < 1e-4  # FP32 drift ~1e-6
+
+
 def test_solver_reuse_and_x0(dev):
     """A second solve on the same solver object gives the same answer; a warm start
     with the solution stops after one iteration."""

@@ -171,10 +171,10 @@ int mlpcg_precond_apply(mlpcg_precond* P, const void* r, void* z, const double* 
 size_t mlpcg_precond_bytes(const mlpcg_precond* P);
 void mlpcg_precond_destroy(mlpcg_precond* P);
 /* No counterpart in the reference (its SM blocks always evaluate the weighted stencil on
- * the fly): for BF16/FP16 plans created after this call, blocks on grids with
+ * the fly): for BF16 plans created after this call, blocks on grids with
  * n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 2, not transposed) recompute K on the matrix
- * cores inside the apply instead of storing it (enable = 1), or store K like every other
- * block (enable = 0, the default).  Process-wide; returns the previous setting. */
+ * cores inside the apply instead of storing it (enable = 1, the default), or store K like
+ * every other block (enable = 0).  Process-wide; returns the previous setting. */
 int mlpcg_set_fused_apply(int enable);
 
 /* ---- PCG solver (cg_tests.py npcg / npcg_flex / npsd, test.py:42-51 predict) ------

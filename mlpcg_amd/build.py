@@ -41,10 +41,15 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (set HIPCC)")
 
 
+# per-source flags: the fused apply keeps MFMA operands and results in VGPRs (no AGPR
+# copies in its inner loop)
+EXTRA = {"fused.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: str, verbose: bool) -> str:
     os.makedirs(OBJ, exist_ok=True)
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *CFLAGS, "-c", os.path.join(SRC, src), "-o", obj]
+    cmd = [hipcc(), *CFLAGS, *EXTRA.get(src, []), "-c", os.path.join(SRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

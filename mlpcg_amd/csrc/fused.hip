@@ -1,26 +1,27 @@
-// Fused SM block apply on the matrix cores (16-bit plans).
+// Fused SM block apply on the matrix cores (BF16 plans).
 //
 //   y(v) = sum_p K_p(v) x(v + d_p),   K_p(v) = b_p + sum_{c,s} W[p,c,s] img[c, v + d_s]
 //
 // (sm_block_3d_kernel.cu:46-70).  Instead of streaming a stored K (54 of the 60 bytes per
-// voxel of the BF16 apply), K is recomputed per voxel as a small GEMM on the MFMA units:
-// for 16 voxels, D[p][v] = sum_k A[p][k] B[k][v] with A = W (27 taps x 27C inputs, padded
-// to 32 x 32*NKC) and B = the image patches of the 16 voxels.  D never leaves registers:
-// each lane multiplies its taps by x(v + d_p) and a butterfly over the four lane groups
-// finishes the sum.  HBM traffic is x, img, y (+ epilogue operands), about 8 B/voxel.
+// voxel of the streamed BF16 apply), both contractions run on the MFMA units, 16 voxels
+// at a time:
+//   1. D[p][v] = b_p + sum_k W[p][k] P[k][v]: W is 27 taps x 27C inputs padded to
+//      32 x 32 NKC, P the image patches of the 16 voxels (an im2col read from LDS);
+//   2. y(v) = sum_p x(v + d_p) D[p][v] as the diagonal of a second 16 x 16 x 32 product
+//      of the x patches with D, which is already in the right lanes after step 1.
+// HBM traffic is x, img and y (+ epilogue operands): about 8 B/voxel instead of 60.
 //
-// Operand layout of v_mfma_f32_16x16x32_{bf16,f16} (lane l, element j / i):
+// Operand layout of v_mfma_f32_16x16x32_{f16,bf16} (lane l, element j / i):
 //   A[l % 16][8 (l / 16) + j],  B[8 (l / 16) + j][l % 16],  D[4 (l / 16) + i][l % 16].
-// Precision: W and img are rounded to the 16-bit type, products accumulate in FP32; for
-// BF16 the weights are split into hi + lo parts (two MFMAs per operand pair), which keeps
-// K as accurate as rounding a FP32-computed K to BF16.  The bias enters as the FP32
-// initial accumulator.
+// Precision: step 1 takes img and W in FP16 (11-bit significands: image values lie in
+// [0, 1] and weights are O(1), so the range is ample) with FP32 accumulation and the bias
+// as the FP32 initial accumulator, which makes K more accurate than a FP32-computed K
+// rounded to BF16; step 2 rounds x and K to BF16 (the storage type of the plan) and
+// accumulates in FP32.
 //
 // Tile: a block of 4 waves covers 8 planes x 8 rows x 64 z-voxels; x and img of the tile
-// plus a one-cell halo are staged in LDS; a wave processes 64 consecutive z-voxels (one
-// per lane at the end) of one (plane, row) per step: 4 MFMA groups of 16 voxels.
-#include <type_traits>
-
+// plus a one-cell halo are staged in LDS; a wave processes 64 consecutive z-voxels of one
+// (plane, row) per step: 4 groups of 16 voxels.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -47,7 +48,6 @@ template <> struct Mfma<f16> {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   }
 };
-
 __host__ __device__ constexpr int nkc(int C) { return (27 * C + 31) / 32; }
 
 // LDS offset of the neighbour d_q (q = 9(d1+1) + 3(d2+1) + (d3+1)) in the halo tile.
@@ -58,24 +58,21 @@ __device__ __forceinline__ int tap_off(int q) {
 }  // namespace
 
 size_t fused_weights_bytes(int C) {
-  return (size_t)2 * 2 * nkc(C) * 64 * 8 * 2 + (size_t)2 * 64 * 4 * sizeof(float);
+  return (size_t)2 * nkc(C) * 64 * 8 * 2 + (size_t)2 * 64 * 4 * sizeof(float);
 }
 
-// apack[part][h][kc][lane][8] (TW), then binit[h][lane][4] (float).
-template <typename TW>
+// apack[h][kc][lane][8] (FP16), then binit[h][lane][4] (float).
 __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __restrict__ W,
                                                               const float* __restrict__ b, int C,
-                                                              int split, TW* __restrict__ apack,
+                                                              f16* __restrict__ apack,
                                                               float* __restrict__ binit) {
   const int NK = nkc(C);
-  const int total = 2 * 2 * NK * 64 * 8;
+  const int total = 2 * NK * 64 * 8;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int j = e % 8, lane = (e / 8) % 64, kc = (e / 512) % NK, h = (e / (512 * NK)) % 2;
-    const int part = e / (1024 * NK);
+    const int j = e % 8, lane = (e / 8) % 64, kc = (e / 512) % NK, h = e / (512 * NK);
     const int m = 16 * h + lane % 16, k = 32 * kc + 8 * (lane / 16) + j;
     const float w = (m < 27 && k < 27 * C) ? W[((long)m * C + k / 27) * 27 + k % 27] : 0.f;
-    const TW hi = (TW)w;
-    apack[e] = part == 0 ? hi : (split ? (TW)(w - (float)hi) : (TW)0.f);
+    apack[e] = (f16)w;
   }
   for (int e = threadIdx.x; e < 2 * 64 * 4; e += blockDim.x) {
     const int i = e % 4, lane = (e / 4) % 64, h = e / 256;
@@ -84,13 +81,20 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
   }
 }
 
-template <typename TW, typename TX, typename TY, typename TP, int C, bool SPLIT>
+// The apply is a second MFMA: after the first, lane (g, n) (g = l / 16, n = l % 16) holds
+// D = K for the taps tau(g, j) = 4g + j (j < 4) and 16 + 4g + j - 4 (j >= 4) of voxel n.
+// With A2[m][k] = x(voxel sigma(m), tap tau(k)) and B2[k][n] = bf16(D[tau(k)][n]),
+// C2[m][n] = sum_k x(sigma(m), tau k) K_{tau k}(n) is y(n) where sigma(m) = n.  The row
+// permutation sigma_t(m) = 4 ((m/4 - t) mod 4) + m % 4 of group t places y(n) of group t
+// in lane (g, n) with g = (n/4 + t) mod 4, element n % 4: over the 4 groups of a 64-voxel
+// chunk every lane ends up holding exactly one voxel, z0 + 16 ((g - n/4) mod 4) + n.
+template <typename TX, typename TY, typename TP, int C>
 __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
-  typedef typename Mfma<TW>::v8 v8;
+  typedef Mfma<bf16>::v8 v8;
+  typedef Mfma<f16>::v8 h8;
   constexpr int NK = nkc(C);
-  constexpr int NP = SPLIT ? 2 : 1;
-  __shared__ float xs[kTile];
-  __shared__ TW is[C * kTile];
+  __shared__ bf16 xs[kTile];
+  __shared__ f16 is[C * kTile];
   if (solve_done(a.done)) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, nn = lane & 15;
   const int tz = a.n3 / kFz, tj = a.n2 / kFj;
@@ -101,41 +105,73 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   const long V = (long)a.n1 * a.n2 * a.n3;
 
   // per-lane constant operands (global loads issued before the tile staging)
-  const TW* apack = (const TW*)a.apack;
-  v8 A[NP][2][NK];
+  const f16* apack = (const f16*)a.apack;
+  h8 A[2][NK];
 #pragma unroll
-  for (int p = 0; p < NP; ++p)
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int kc = 0; kc < NK; ++kc)
-        A[p][h][kc] = *(const v8*)(apack + ((((long)p * 2 + h) * NK + kc) * 64 + lane) * 8);
+    for (int kc = 0; kc < NK; ++kc) A[h][kc] = *(const h8*)(apack + (((long)h * NK + kc) * 64 + lane) * 8);
   const float* binit = a.binit;
-  f32x4 B0 = *(const f32x4*)(binit + lane * 4);
-  f32x4 B1 = *(const f32x4*)(binit + (64 + lane) * 4);
+  const f32x4 B0 = *(const f32x4*)(binit + lane * 4);
+  const f32x4 B1 = *(const f32x4*)(binit + (64 + lane) * 4);
   const float scale = a.in_scale ? (float)(*a.in_scale) : 1.f;
   const float c0 = a.prev ? *(const float*)a.c0 : 0.f;
   const float c1 = a.prev ? *(const float*)a.c1 : 0.f;
 
-  // stage x (as FP32) and img of the halo tile; zero outside the grid
+  // stage x and img of the halo tile (BF16); zero outside the grid.  Branch-free: every
+  // lane loads from a clamped in-grid address and selects zero, so all loads of a batch
+  // are in flight together before the LDS writes.  Rows r = w + 4 k (k < 25) of the
+  // 10 x 10 rows; element e = lane of each row, and e = 64 + lane for lanes 0 and 1.
   const TX* __restrict__ x = (const TX*)a.x;
-  const TW* __restrict__ img = (const TW*)a.img;
-  for (int r = w; r < kPi * kPj; r += 4) {
-    const int pi = r / kPj, pj = r - pi * kPj;
-    const int gi = i0 + pi - 1, gj = j0 + pj - 1;
-    const bool rok = gi >= 0 && gi < a.n1 && gj >= 0 && gj < a.n2;
-    const long rb = ((long)gi * a.n2 + gj) * a.n3;
-    for (int e = lane; e < kPz; e += 64) {
-      const int gz = z0 - 1 + e;
-      const bool ok = rok && gz >= 0 && gz < a.n3;
-      xs[r * kPz + e] = ok ? (float)x[rb + gz] : 0.f;
+  const f16* __restrict__ img = (const f16*)a.img;
+  static_assert(kPi * kPj == 100 && kPz == 66, "staging assumes 100 rows of 66 cells");
 #pragma unroll
-      for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + e] = ok ? img[c * V + rb + gz] : (TW)0.f;
+  for (int k0 = 0; k0 < 25; k0 += 13) {
+    constexpr int KB = 13;
+    bf16 xv[KB][2];
+    f16 iv[KB][C][2];
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+      const int r = w + 4 * (k0 + kk);
+      if (k0 + kk < 25) {
+        const int pi = r / kPj, pj = r - pi * kPj;
+        const int gi = i0 + pi - 1, gj = j0 + pj - 1;
+        const bool rok = gi >= 0 && gi < a.n1 && gj >= 0 && gj < a.n2;
+        const long rb = ((long)min(max(gi, 0), a.n1 - 1) * a.n2 + min(max(gj, 0), a.n2 - 1)) * a.n3;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int e = u == 0 ? lane : 64 + (lane & 1);
+          const int gz = z0 - 1 + e;
+          const bool ok = rok && gz >= 0 && gz < a.n3;
+          const long gv = rb + min(max(gz, 0), a.n3 - 1);
+          const bf16 xr = (bf16)x[gv];
+          xv[kk][u] = ok ? xr : (bf16)0.f;
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            const f16 ir = img[c * V + gv];
+            iv[kk][c][u] = ok ? ir : (f16)0.f;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+      const int r = w + 4 * (k0 + kk);
+      if (k0 + kk < 25) {
+        xs[r * kPz + lane] = xv[kk][0];
+#pragma unroll
+        for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + lane] = iv[kk][c][0];
+        if (lane < 2) {
+          xs[r * kPz + 64 + lane] = xv[kk][1];
+#pragma unroll
+          for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + 64 + lane] = iv[kk][c][1];
+        }
+      }
     }
   }
 
-  // per-lane LDS offsets: image inputs k of this lane's B rows, taps of its D rows
-  int boff[NK][8], xoff[2][4];
+  // per-lane LDS offsets: image inputs k of this lane's B rows; taps tau(g, j) of its D rows
+  int boff[NK][8], xoff[8];
 #pragma unroll
   for (int kc = 0; kc < NK; ++kc)
 #pragma unroll
@@ -144,52 +180,79 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
       boff[kc][j] = k < 27 * C ? (k / 27) * kTile + tap_off(k % 27) : 0;  // padding reads any cell
     }
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+  for (int j = 0; j < 8; ++j) {
+    const int m = j < 4 ? 4 * g + j : 16 + 4 * g + j - 4;
+    xoff[j] = m < 27 ? tap_off(m) : 0;  // rows m >= 27 of D are exactly zero
+  }
+  // x rows of group t are read for voxel sigma_t(nn); this lane's result is group tstar
+  int xsig[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int m = 16 * h + 4 * g + i;
-      xoff[h][i] = m < 27 ? tap_off(m) : 0;  // rows m >= 27 of D are exactly zero
-    }
+  for (int tt = 0; tt < 4; ++tt) xsig[tt] = 16 * tt + 4 * (((nn >> 2) - tt) & 3) + (nn & 3);
+  const int tstar = (g - (nn >> 2)) & 3;
+  const int myz = 16 * tstar + nn;
   __syncthreads();
 
   TY* __restrict__ y = (TY*)a.y;
-  const TP* __restrict__ prev = (const TP*)a.prev;
+  // epilogue operands through always-valid pointers (x has at least 2V bytes), so the
+  // loads are unconditional and can be issued one chunk ahead
+  const uint8_t* mA = a.mask_a ? a.mask_a : (const uint8_t*)a.x;
+  const uint8_t* mB = a.mask_b ? a.mask_b : (const uint8_t*)a.x;
+  const TP* pP = a.prev ? (const TP*)a.prev : (const TP*)a.x;
+  const bool has_a = a.mask_a != nullptr, has_b = a.mask_b != nullptr, has_p = a.prev != nullptr;
+  const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
+  auto vox = [&](int q) {
+    const int ii = q >> 1, jj = w + 4 * (q & 1);
+    return ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0 + myz;
+  };
+  long v = vox(0);
+  uint8_t ka = mA[v], kb = mB[v];
+  TP pv = pP[v];
   for (int q = 0; q < kFi * 2; ++q) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
     const int cb = ((ii + 1) * kPj + (jj + 1)) * kPz + 1;
-    const long v = ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0 + lane;
-    // epilogue operands first: their latency overlaps the MFMA work
-    const uint8_t ka = a.mask_a ? a.mask_a[v] : (uint8_t)1;
-    const uint8_t kb = a.mask_b ? a.mask_b[v] : (uint8_t)1;
-    const float pv = prev ? (float)prev[v] : 0.f;
-    float ymine = 0.f;
+    // next chunk's epilogue operands
+    const long vn = vox(q + 1 < kFi * 2 ? q + 1 : q);
+    const uint8_t ka_n = mA[vn], kb_n = mB[vn];
+    const TP pv_n = pP[vn];
+    f32x4 c2[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) {
       const int ci = cb + 16 * tt + nn;
       f32x4 d0 = B0, d1 = B1;
 #pragma unroll
       for (int kc = 0; kc < NK; ++kc) {
-        v8 bv;
+        h8 bv;
 #pragma unroll
         for (int j = 0; j < 8; ++j) bv[j] = is[ci + boff[kc][j]];
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          d0 = Mfma<TW>::run(A[p][0][kc], bv, d0);
-          d1 = Mfma<TW>::run(A[p][1][kc], bv, d1);
-        }
+        d0 = Mfma<f16>::run(A[0][kc], bv, d0);
+        d1 = Mfma<f16>::run(A[1][kc], bv, d1);
       }
-      float yp = 0.f;
+      const int cx = cb + xsig[tt];
+      v8 ax, bk;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) yp += d0[i] * xs[ci + xoff[0][i]] + d1[i] * xs[ci + xoff[1][i]];
-      yp += __shfl_xor(yp, 16, 64);
-      yp += __shfl_xor(yp, 32, 64);
-      ymine = (g == tt) ? yp : ymine;
+      for (int j = 0; j < 8; ++j) ax[j] = xs[cx + xoff[j]];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        bk[j] = (bf16)d0[j];
+        bk[j + 4] = (bf16)d1[j];
+      }
+      c2[tt] = Mfma<bf16>::run(ax, bk, zero4);
     }
+    // this lane's voxel: group tstar, element nn % 4 (selects, no branches)
+    f32x4 cs = c2[0];
+#pragma unroll
+    for (int tt = 1; tt < 4; ++tt) cs = (tstar == tt) ? c2[tt] : cs;
+    const int e4 = nn & 3;
+    const float ymine = e4 == 0 ? cs[0] : (e4 == 1 ? cs[1] : (e4 == 2 ? cs[2] : cs[3]));
     float val = ymine * scale;
-    if (ka == 0) val = 0.f;
-    if (prev) val = c0 * pv + c1 * val;
-    if (kb == 0) val = 0.f;
+    if (has_a && ka == 0) val = 0.f;
+    if (has_p) val = c0 * (float)pv + c1 * val;
+    if (has_b && kb == 0) val = 0.f;
     y[v] = (TY)val;
+    v = vn;
+    ka = ka_n;
+    kb = kb_n;
+    pv = pv_n;
   }
 }
 
@@ -199,45 +262,32 @@ bool fused_eligible(int C, int B, int n1, int n2, int n3) {
 
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
   MLPCG_REQUIRE(C == 1 || C == 2, "fused_pack: C must be 1 or 2");
-  float* binit = (float*)((char*)wbuf + (size_t)2 * 2 * nkc(C) * 64 * 8 * 2);
-  if (tw == kBF16)
-    fused_pack_kernel<bf16><<<1, kThreads, 0, st>>>((const float*)W, (const float*)b, C, 1, (bf16*)wbuf, binit);
-  else if (tw == kF16)
-    fused_pack_kernel<f16><<<1, kThreads, 0, st>>>((const float*)W, (const float*)b, C, 0, (f16*)wbuf, binit);
-  else
-    MLPCG_REQUIRE(false, "fused_pack: 16-bit type expected (got %d)", tw);
+  MLPCG_REQUIRE(tw == kBF16, "fused_pack: BF16 plans only (got %d)", tw);
+  float* binit = (float*)((char*)wbuf + (size_t)2 * nkc(C) * 64 * 8 * 2);
+  fused_pack_kernel<<<1, kThreads, 0, st>>>((const float*)W, (const float*)b, C, (f16*)wbuf, binit);
   return check_launch("fused_pack");
 }
 
-template <typename TW, typename TX, typename TY, int C>
+template <typename TX, typename TY, int C>
 static void launch_fused_t(const FusedArgs& a, hipStream_t st) {
-  constexpr bool split = std::is_same<TW, bf16>::value;
   const unsigned nb = (unsigned)((a.n1 / kFi) * (a.n2 / kFj) * (a.n3 / kFz));
-  sm_fused_kernel<TW, TX, TY, TW, C, split><<<nb, kThreads, 0, st>>>(a);
+  sm_fused_kernel<TX, TY, bf16, C><<<nb, kThreads, 0, st>>>(a);
 }
 
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& args, hipStream_t st) {
+  MLPCG_REQUIRE(tw == kBF16, "sm_fused: BF16 plans only (got %d)", tw);
   MLPCG_REQUIRE(fused_eligible(C, 1, args.n1, args.n2, args.n3),
                 "sm_fused: grid (%d,%d,%d) / C=%d not supported", args.n1, args.n2, args.n3, C);
-  FusedArgs a = args;
-#define MLPCG_FU(TW, TXX, TYY)                                 \
-  do {                                                         \
-    if (C == 1) launch_fused_t<TW, TXX, TYY, 1>(a, st);        \
-    else launch_fused_t<TW, TXX, TYY, 2>(a, st);               \
+  const FusedArgs& a = args;
+#define MLPCG_FU(TXX, TYY)                                 \
+  do {                                                     \
+    if (C == 1) launch_fused_t<TXX, TYY, 1>(a, st);        \
+    else launch_fused_t<TXX, TYY, 2>(a, st);               \
   } while (0)
-  if (tw == kBF16) {
-    if (tx == kF32 && ty == kBF16) MLPCG_FU(bf16, float, bf16);
-    else if (tx == kBF16 && ty == kF32) MLPCG_FU(bf16, bf16, float);
-    else if (tx == kBF16 && ty == kBF16) MLPCG_FU(bf16, bf16, bf16);
-    else MLPCG_REQUIRE(false, "sm_fused: vector types (%d,%d) unsupported", tx, ty);
-  } else if (tw == kF16) {
-    if (tx == kF32 && ty == kF16) MLPCG_FU(f16, float, f16);
-    else if (tx == kF16 && ty == kF32) MLPCG_FU(f16, f16, float);
-    else if (tx == kF16 && ty == kF16) MLPCG_FU(f16, f16, f16);
-    else MLPCG_REQUIRE(false, "sm_fused: vector types (%d,%d) unsupported", tx, ty);
-  } else {
-    MLPCG_REQUIRE(false, "sm_fused: 16-bit stencil type expected (got %d)", tw);
-  }
+  if (tx == kF32 && ty == kBF16) MLPCG_FU(float, bf16);
+  else if (tx == kBF16 && ty == kF32) MLPCG_FU(bf16, float);
+  else if (tx == kBF16 && ty == kBF16) MLPCG_FU(bf16, bf16);
+  else MLPCG_REQUIRE(false, "sm_fused: vector types (%d,%d) unsupported", tx, ty);
 #undef MLPCG_FU
   return check_launch("sm_fused");
 }

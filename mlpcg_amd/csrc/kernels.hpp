@@ -37,9 +37,9 @@ struct UpArgs {
   int B, n1, n2, n3;
 };
 
-// Fused apply for 16-bit plans (fused.hip): K recomputed on the matrix cores from the
-// 16-bit image img [C][V] and the packed weights (launch_fused_pack), applied to x with the
-// same epilogue as ApplyArgs.  Grids with n1, n2 % 8 == 0 and n3 % 64 == 0, B = 1, C <= 2.
+// Fused apply for BF16 plans (fused.hip): K recomputed on the matrix cores from the FP16
+// image img [C][V] and the packed weights (launch_fused_pack), applied to x with the same
+// epilogue as ApplyArgs.  Grids with n1, n2 % 8 == 0 and n3 % 64 == 0, B = 1, C <= 2.
 struct FusedArgs {
   const void* img;
   const void* apack;

@@ -23,7 +23,7 @@ static size_t dsize(int dt) {
   }
 }
 
-static int g_fused_apply = 0;
+static int g_fused_apply = 1;
 void set_fused_apply(int enable) { g_fused_apply = enable ? 1 : 0; }
 int fused_apply_enabled() { return g_fused_apply; }
 
@@ -108,7 +108,7 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
     P->blocks.push_back({n, n, 1, 0});                                   // l_t
     P->nw = n + 1;
   }
-  const bool half = (dtype == kBF16 || dtype == kF16);
+  const bool half = (dtype == kBF16);
   for (Precond::Blk& b : P->blocks) {
     const int* d = P->dims[b.level];
     void* K = nullptr;
@@ -151,7 +151,7 @@ int Precond::update(const void* image, const void* const* weights, const void* c
                         nullptr, st));
   if (mask) TRY(launch_nonzero_mask(acc, (const char*)img[0] + (size_t)V[0] * as, maskimg, V[0], st));
   for (int l = 0; l <= n; ++l)
-    if (imgw[l]) TRY(launch_cast(acc, dtype, img[l], imgw[l], (long)C * V[l], st));
+    if (imgw[l]) TRY(launch_cast(acc, kF16, img[l], imgw[l], (long)C * V[l], st));  // FP16 MFMA input
   for (size_t k = 0; k < blocks.size(); ++k) {
     const Blk& b = blocks[k];
     const int* d = dims[b.level];

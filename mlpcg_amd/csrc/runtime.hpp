@@ -39,7 +39,7 @@ struct Precond {
   };
   std::vector<Blk> blocks;          // one per entry of K (nullptr for fused blocks)
   std::vector<void*> wpack;         // fused blocks: packed MFMA weights + bias init
-  void* imgw[kMaxLevels + 1] = {};  // 16-bit image of levels with fused blocks
+  void* imgw[kMaxLevels + 1] = {};  // FP16 image of levels with fused blocks
   int nw = 0;                       // number of block weights before c0/c1
   void* lin_ws = nullptr;           // SMLinear reduction workspace
   void* ybar = nullptr;             // SMLinear mean patch scratch
@@ -54,7 +54,7 @@ struct Precond {
 };
 
 // Plans created while this is set use the fused matrix-core apply for eligible 16-bit
-// blocks (process-wide; default off).
+// blocks (process-wide; default on).
 void set_fused_apply(int enable);
 int fused_apply_enabled();
 

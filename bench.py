@@ -9,10 +9,13 @@ SMLinear coefficients) followed by npsd (or --method) to ||b - Ax||/||b|| <= 1e-
 residual recomputed in FP64 every iteration.  `value` is the mean seconds per step with
 the BF16 preconditioner; inputs are resident in HBM before timing.
 
-roofline.traffic is the HBM bytes per launch of the dominant kernel read by two
-rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, calibrated on a 1 GiB copy); the
-passes are recorded in profiles/sm_apply_bf16_level0_pillars256_hbm_traffic.txt and
-apply to the 256^3 workload only (null otherwise).
+roofline is that of the dominant kernel of the BF16 solve, the fused matrix-core
+level-0 block; roofline_streamed that of the stored-stencil level-0 apply (used by the
+FP32 plan).  Their traffic fields are the HBM bytes per launch read by two rocprofv3
+counter passes (FETCH_SIZE, WRITE_SIZE, calibrated on a 1 GiB copy), recorded in
+profiles/sm_fused_bf16_level0_pillars256_hbm_traffic.txt and
+profiles/sm_apply_bf16_level0_pillars256_hbm_traffic.txt for the 256^3 workload only
+(null otherwise).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -38,14 +41,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_RECORD = os.path.join(ROOT, "profiles", "sm_apply_bf16_level0_pillars256_hbm_traffic.txt")
+TRAFFIC_FUSED = os.path.join(ROOT, "profiles", "sm_fused_bf16_level0_pillars256_hbm_traffic.txt")
+TRAFFIC_STREAMED = os.path.join(ROOT, "profiles", "sm_apply_bf16_level0_pillars256_hbm_traffic.txt")
 
 
-def recorded_traffic(n):
-    """HBM bytes per launch of the level-0 BF16 apply at 256^3 from the counter passes."""
-    if n != 256 or not os.path.exists(TRAFFIC_RECORD):
+def recorded_traffic(n, path):
+    """HBM bytes per launch of a level-0 kernel at 256^3 from the recorded counter passes."""
+    if n != 256 or not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_RECORD):
+    for line in open(path):
         if line.startswith("traffic_bytes_per_launch"):
             return int(line.split("=")[1])
     return None
@@ -129,6 +133,31 @@ def roofline_apply(model, image, r, dtype, iters):
     esz = torch.tensor([], dtype=dtype).element_size()
     alg = 27 * esz * V + 4 * V + esz * V
     return t, alg
+
+
+# K contraction 2*27*(27C+1) (bias as the constant input) + apply 2*27, C = 1
+FUSED_FLOP_PER_VOXEL = 2 * 27 * 28 + 2 * 27
+
+
+def roofline_fused(plan, r, iters):
+    """Level-0 pre-smoother block of a BF16 plan alone (block 0: FP32 input scaled by
+    1/||r||, BF16 output; the fused matrix-core kernel), timed with HIP events on torch's
+    current stream, which is the stream the library launches on.  Algorithmic bytes per
+    voxel: x 4 + FP16 image 2 + y 2."""
+    import torch
+    scale = torch.tensor(1.0 / float(r.norm()), dtype=torch.float64, device=r.device)
+    x = r.reshape(r.shape[-3:]).contiguous()
+    for _ in range(3):
+        plan.apply_block(0, x, torch.bfloat16, scale)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        plan.apply_block(0, x, torch.bfloat16, scale)
+    e1.record()
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / iters
+    V = x.numel()
+    return t, 8 * V, FUSED_FLOP_PER_VOXEL * V
 
 
 def copy_bandwidth(dev, nbytes=2 << 30):
@@ -332,8 +361,13 @@ def main():
     r0 = b.reshape(sc.shape)
 
     if args.profile_only:
-        # the dominant kernel, then a plain 1 GiB device copy whose counter readings
-        # calibrate the FETCH_SIZE / WRITE_SIZE counters against known bytes
+        # the dominant kernel (fused level-0 block), the stored-stencil apply, then a plain
+        # 1 GiB device copy whose counter readings calibrate FETCH_SIZE / WRITE_SIZE
+        plan = model.make_preconditioner(image, torch.bfloat16)
+        t, alg, _ = roofline_fused(plan, r0, args.roofline_iters)
+        log(f"[bench] profile-only: sm_fused bf16 {t * 1e3:.3f} ms, {alg / t / 1e9:.1f} GB/s, "
+            f"algorithmic bytes {alg}")
+        del plan
         t, alg = roofline_apply(model, image, r0, torch.bfloat16, args.roofline_iters)
         log(f"[bench] profile-only: sm_apply bf16 {t * 1e3:.3f} ms, {alg / t / 1e9:.1f} GB/s, "
             f"algorithmic bytes {alg}")
@@ -398,16 +432,30 @@ def main():
         results[name]["precond_rel_err_vs_fp64"] = float((z - z64).norm() / z64.norm())
     del plan64, z64
 
-    # roofline of the dominant kernel (level-0 SM apply, BF16)
-    t_k, alg = roofline_apply(model, image, r0, torch.bfloat16, args.roofline_iters)
+    # roofline of the dominant kernel: the fused level-0 block of the BF16 plan.  Its
+    # arithmetic intensity (FUSED_FLOP_PER_VOXEL / 8 B = 196 FLOP/B) is below the
+    # BF16/FP16 ridge point (2.5 PFLOP/s / 8 TB/s = 312), so its roofline is HBM.
+    t_k, alg, flop = roofline_fused(plans["bf16"], r0, args.roofline_iters)
     achieved = alg / t_k / 1e9
     copy_gbs = copy_bandwidth(dev)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": recorded_traffic(args.n),
-            "kernel": "sm_apply_kernel<bf16,f32,bf16,bf16,VZ=8> level 0",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": recorded_traffic(args.n, TRAFFIC_FUSED),
+            "kernel": "sm_fused_kernel<f32 x, bf16 y, C=1> level 0 (pre.0)",
             "avg_ms": round(t_k * 1e3, 4), "algorithmic_bytes": alg,
+            "algorithmic_tflops": round(flop / t_k / 1e12, 1),
+            "mfma_frac": round(flop / t_k / 2.5e15, 4),
             "measured_copy_GBs": round(copy_gbs, 1)}
-    log(f"[bench] roofline: {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of peak), copy {copy_gbs:.0f} GB/s")
+    log(f"[bench] roofline fused: {t_k * 1e6:.1f} us, {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of HBM), "
+        f"{flop / t_k / 1e12:.0f} TFLOP/s")
+    t_s, alg_s = roofline_apply(model, image, r0, torch.bfloat16, args.roofline_iters)
+    ach_s = alg_s / t_s / 1e9
+    roof_streamed = {"bound": "hbm", "achieved": round(ach_s, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach_s / HBM_PEAK_GBS, 4),
+                     "traffic": recorded_traffic(args.n, TRAFFIC_STREAMED),
+                     "kernel": "sm_apply_kernel<bf16,f32,bf16,bf16,VZ=8> level 0 (stored BF16 stencils)",
+                     "avg_ms": round(t_s * 1e3, 4), "algorithmic_bytes": alg_s}
+    log(f"[bench] roofline streamed: {t_s * 1e6:.1f} us, {ach_s:.0f} GB/s ({ach_s / HBM_PEAK_GBS:.1%} of peak), "
+        f"copy {copy_gbs:.0f} GB/s")
 
     train = None
     if not args.skip_train:
@@ -440,6 +488,7 @@ def main():
                    "grid": args.n, "levels": args.levels, "method": args.method,
                    "weights": wsrc, "graphs": bool(args.graphs), "parallelism": "single GPU"},
         "roofline": roof,
+        "roofline_streamed": roof_streamed,
         "cpu_baseline": cpu,
         "fp32_s_per_solve": round(f32["s_per_solve"], 6),
         "bf16_speedup_vs_fp32": round(f32["s_per_solve"] / bf["s_per_solve"], 3),

@@ -176,6 +176,12 @@ void mlpcg_precond_destroy(mlpcg_precond* P);
  * cores inside the apply instead of storing it (enable = 1, the default), or store K like
  * every other block (enable = 0).  Process-wide; returns the previous setting. */
 int mlpcg_set_fused_apply(int enable);
+/* No counterpart in the reference (it has no plan object): y = block(x * (*in_scale)) for
+ * stencil block `block` of the plan alone (block order of the weights array above; not a
+ * transposed one), on the grid of its level, without level combine or masks; x and y of
+ * the plan's storage type or its vector type.  For measurement and tests. */
+int mlpcg_precond_apply_block(mlpcg_precond* P, int block, const void* x, int x_dtype, void* y,
+                              int y_dtype, const double* in_scale, void* stream);
 
 /* ---- PCG solver (cg_tests.py npcg / npcg_flex / npsd, test.py:42-51 predict) ------
  * flags solver: vectors live on the full grid (length V, zero off the fluid cells),

@@ -310,6 +310,12 @@ void mlpcg_precond_destroy(mlpcg_precond* P) {
   delete P;
 }
 
+int mlpcg_precond_apply_block(mlpcg_precond* P, int block, const void* x, int x_dtype, void* y,
+                              int y_dtype, const double* in_scale, void* stream) {
+  MLPCG_REQUIRE(P && P->impl && x && y, "precond_apply_block: null pointer");
+  return P->impl->apply_block(block, x, x_dtype, y, y_dtype, in_scale, STREAM(stream));
+}
+
 int mlpcg_set_fused_apply(int enable) {
   const int prev = fused_apply_enabled();
   set_fused_apply(enable);

@@ -176,6 +176,59 @@ int Precond::update(const void* image, const void* const* weights, const void* c
   return 0;
 }
 
+// One stencil block: the fused matrix-core kernel or the stored-stencil apply, with the
+// level-combine / mask epilogue of ApplyArgs.
+int Precond::run_block(int k, int level, const void* x, int tx, void* y, int ty, const void* prev,
+                       const void* c0, const void* c1, const uint8_t* ma, const uint8_t* mb,
+                       const double* sc, const int* done, hipStream_t st) {
+  if (blocks[k].fused) {
+    FusedArgs f;
+    f.img = imgw[level];
+    f.apack = wpack[k];
+    f.binit = (const float*)((const char*)wpack[k] + fused_weights_bytes(C) - 2 * 64 * 4 * sizeof(float));
+    f.x = x;
+    f.y = y;
+    f.prev = prev;
+    f.c0 = c0;
+    f.c1 = c1;
+    f.mask_a = ma;
+    f.mask_b = mb;
+    f.in_scale = sc;
+    f.done = done;
+    f.n1 = dims[level][0];
+    f.n2 = dims[level][1];
+    f.n3 = dims[level][2];
+    return launch_sm_fused(dtype, tx, ty, C, f, st);
+  }
+  ApplyArgs a;
+  a.K = K[k];
+  a.x = x;
+  a.y = y;
+  a.prev = prev;
+  a.c0 = c0;
+  a.c1 = c1;
+  a.mask_a = ma;
+  a.mask_b = mb;
+  a.in_scale = sc;
+  a.done = done;
+  a.B = 1;
+  a.n1 = dims[level][0];
+  a.n2 = dims[level][1];
+  a.n3 = dims[level][2];
+  return launch_sm_apply(dtype, tx, ty, a, st);
+}
+
+int Precond::apply_block(int k, const void* x, int tx, void* y, int ty, const double* in_scale,
+                         hipStream_t st) {
+  MLPCG_REQUIRE(k >= 0 && k < (int)blocks.size(), "precond_apply_block: block %d out of range", k);
+  MLPCG_REQUIRE(!blocks[k].trans, "precond_apply_block: block %d is a transposed stencil", k);
+  MLPCG_REQUIRE((tx == dtype || tx == (dtype == kF64 ? kF64 : kF32)) &&
+                    (ty == dtype || ty == (dtype == kF64 ? kF64 : kF32)),
+                "precond_apply_block: vector types (%d,%d) do not fit the plan type %d", tx, ty, dtype);
+  return run_block(k, blocks[k].level, x, tx, y, ty, nullptr, nullptr, nullptr, nullptr, nullptr, in_scale,
+                   nullptr, st);
+}
+
 int Precond::apply(const float* r_in, float* z_out, const double* in_scale, const uint8_t* out_mask,
                    const int* done, hipStream_t st) {
   const int vdt = dtype == kF64 ? kF64 : kF32;  // type of the solver-side vectors
@@ -185,41 +238,7 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
   auto apply_blk = [&](int k, int level, const void* x, int tx, void* y, int ty, const void* prev,
                        const void* c0, const void* c1, const uint8_t* ma, const uint8_t* mb,
                        const double* sc) {
-    if (blocks[k].fused) {
-      FusedArgs f;
-      f.img = imgw[level];
-      f.apack = wpack[k];
-      f.binit = (const float*)((const char*)wpack[k] + fused_weights_bytes(C) - 2 * 64 * 4 * sizeof(float));
-      f.x = x;
-      f.y = y;
-      f.prev = prev;
-      f.c0 = c0;
-      f.c1 = c1;
-      f.mask_a = ma;
-      f.mask_b = mb;
-      f.in_scale = sc;
-      f.done = done;
-      f.n1 = dims[level][0];
-      f.n2 = dims[level][1];
-      f.n3 = dims[level][2];
-      return launch_sm_fused(dtype, tx, ty, C, f, st);
-    }
-    ApplyArgs a;
-    a.K = K[k];
-    a.x = x;
-    a.y = y;
-    a.prev = prev;
-    a.c0 = c0;
-    a.c1 = c1;
-    a.mask_a = ma;
-    a.mask_b = mb;
-    a.in_scale = sc;
-    a.done = done;
-    a.B = 1;
-    a.n1 = dims[level][0];
-    a.n2 = dims[level][1];
-    a.n3 = dims[level][2];
-    return launch_sm_apply(dtype, tx, ty, a, st);
+    return run_block(k, level, x, tx, y, ty, prev, c0, c1, ma, mb, sc, done, st);
   };
   auto pool = [&](const void* in, void* outp, int fine_level) {
     const int* d = dims[fine_level];

@@ -50,6 +50,11 @@ struct Precond {
              hipStream_t st);
   int apply(const float* r, float* z, const double* in_scale, const uint8_t* out_mask,
             const int* done, hipStream_t st);
+  int apply_block(int k, const void* x, int tx, void* y, int ty, const double* in_scale,
+                  hipStream_t st);
+  int run_block(int k, int level, const void* x, int tx, void* y, int ty, const void* prev,
+                const void* c0, const void* c1, const uint8_t* ma, const uint8_t* mb,
+                const double* sc, const int* done, hipStream_t st);
   ~Precond();
 };
 

@@ -88,6 +88,17 @@ class Preconditioner:
     def nbytes(self) -> int:
         return int(lib.mlpcg_precond_bytes(self._h))
 
+    def apply_block(self, k, x, out_dtype=None, in_scale=None):
+        """y = block k (weights-array order) applied to x alone, on its level's grid; the
+        fused matrix-core kernel when the plan uses it for that block."""
+        out_dtype = out_dtype or x.dtype
+        x = x.contiguous()
+        y = torch.empty(x.shape, dtype=out_dtype, device=x.device)
+        check(lib.mlpcg_precond_apply_block(self._h, int(k), ptr(x), code_of(x.dtype), ptr(y),
+                                            code_of(out_dtype), ptr(in_scale),
+                                            stream_ptr(self.device)), "precond_apply_block")
+        return y
+
     def apply(self, r, in_scale=None, out_mask=None, out=None):
         """r: (..., N1, N2, N3) tensor of the vector type (float32; float64 for a float64
         plan).  in_scale: optional 0-dim float64 device tensor multiplying r."""

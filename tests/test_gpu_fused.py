@@ -82,10 +82,13 @@ def test_fused_matches_stored_stencils(dev, C, mask, n, shape):
     r = torch.randint(-4, 5, shape, generator=g).float().cuda()
     pf, ps = _plan(m, img, dtype, True), _plan(m, img, dtype, False)
     assert pf.nbytes < ps.nbytes  # fused blocks keep no stencils
+    # the level-0 pre block alone: exact sums on both paths, so bit-identical
+    sc = torch.tensor(0.5, dtype=torch.float64, device=dev)
+    yf, ys = pf.apply_block(0, r, dtype, sc), ps.apply_block(0, r, dtype, sc)
+    assert yf.dtype == dtype and torch.equal(yf, ys)
     zf, zs = pf.apply(r), ps.apply(r)
     assert torch.isfinite(zf).all()
     assert _nrel(zf, zs) < 1e-5
-    sc = torch.tensor(0.5, dtype=torch.float64, device=dev)
     om = (img[0] > 0.3).to(torch.uint8)
     zf2, zs2 = pf.apply(r, in_scale=sc, out_mask=om), ps.apply(r, in_scale=sc, out_mask=om)
     assert _nrel(zf2, zs2) < 1e-5

@@ -173,9 +173,10 @@ void mlpcg_precond_destroy(mlpcg_precond* P);
 /* No counterpart in the reference (its SM blocks always evaluate the weighted stencil on
  * the fly): for BF16 plans created after this call, blocks on grids with
  * n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 2, not transposed) recompute K on the matrix
- * cores inside the apply instead of storing it (enable = 1, the default), or store K like
- * every other block (enable = 0).  Process-wide; returns the previous setting. */
-int mlpcg_set_fused_apply(int enable);
+ * cores inside the apply instead of storing it.  mode 0: never (K stored like every other
+ * block); 1 (the default): on such grids of at least 2^20 voxels (smaller ones are faster
+ * with stored K); 2: on every such grid.  Process-wide; returns the previous mode. */
+int mlpcg_set_fused_apply(int mode);
 /* No counterpart in the reference (it has no plan object): y = block(x * (*in_scale)) for
  * stencil block `block` of the plan alone (block order of the weights array above; not a
  * transposed one), on the grid of its level, without level combine or masks; x and y of

@@ -3,7 +3,7 @@
 #include <cstdio>
 #include <cstring>
 
-#include "../../include/mlpcg.h"
+#include "mlpcg.h"
 #include "runtime.hpp"
 
 namespace mlpcg {
@@ -316,9 +316,9 @@ int mlpcg_precond_apply_block(mlpcg_precond* P, int block, const void* x, int x_
   return P->impl->apply_block(block, x, x_dtype, y, y_dtype, in_scale, STREAM(stream));
 }
 
-int mlpcg_set_fused_apply(int enable) {
+int mlpcg_set_fused_apply(int mode) {
   const int prev = fused_apply_enabled();
-  set_fused_apply(enable);
+  set_fused_apply(mode);
   return prev;
 }
 

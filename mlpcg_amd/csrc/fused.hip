@@ -199,6 +199,11 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   const uint8_t* mB = a.mask_b ? a.mask_b : (const uint8_t*)a.x;
   const TP* pP = a.prev ? (const TP*)a.prev : (const TP*)a.x;
   const bool has_a = a.mask_a != nullptr, has_b = a.mask_b != nullptr, has_p = a.prev != nullptr;
+  const bool dots = a.partials != nullptr;
+  const float* d0p = a.dot0 ? a.dot0 : (const float*)a.x;
+  const float* d1p = a.dot1 ? a.dot1 : (const float*)a.x;
+  const long dmask = a.dot1 ? ~0L : 0L;  // without dot1 every lane reads element 0
+  double red[2] = {0.0, 0.0};
   const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
   auto vox = [&](int q) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
@@ -207,6 +212,7 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   long v = vox(0);
   uint8_t ka = mA[v], kb = mB[v];
   TP pv = pP[v];
+  float da = dots ? d0p[v] : 0.f, db = dots ? d1p[v & dmask] : 0.f;
   for (int q = 0; q < kFi * 2; ++q) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
     const int cb = ((ii + 1) * kPj + (jj + 1)) * kPz + 1;
@@ -214,6 +220,7 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     const long vn = vox(q + 1 < kFi * 2 ? q + 1 : q);
     const uint8_t ka_n = mA[vn], kb_n = mB[vn];
     const TP pv_n = pP[vn];
+    const float da_n = dots ? d0p[vn] : 0.f, db_n = dots ? d1p[vn & dmask] : 0.f;
     f32x4 c2[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) {
@@ -248,13 +255,23 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     if (has_a && ka == 0) val = 0.f;
     if (has_p) val = c0 * (float)pv + c1 * val;
     if (has_b && kb == 0) val = 0.f;
-    y[v] = (TY)val;
+    const TY yv = (TY)val;
+    y[v] = yv;
+    if (dots) {
+      red[0] += (double)(float)yv * (double)da;
+      red[1] += (double)(float)yv * (double)db;
+    }
     v = vn;
     ka = ka_n;
     kb = kb_n;
     pv = pv_n;
+    da = da_n;
+    db = db_n;
   }
+  if (dots) block_partials<2>(red, a.partials);
 }
+
+unsigned fused_blocks(int n1, int n2, int n3) { return (unsigned)((n1 / kFi) * (n2 / kFj) * (n3 / kFz)); }
 
 bool fused_eligible(int C, int B, int n1, int n2, int n3) {
   return (C == 1 || C == 2) && B == 1 && n1 % kFi == 0 && n2 % kFj == 0 && n3 % kFz == 0;

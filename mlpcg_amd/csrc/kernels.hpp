@@ -53,10 +53,16 @@ struct FusedArgs {
   const uint8_t* mask_b;
   const double* in_scale;
   const int* done;
+  // optional: per-block partial sums [2][blocks] (doubles) of y.dot0 and y.dot1 over the
+  // full grid (dot1 may be null), for a following finalize (solver dot products on z)
+  const float* dot0;
+  const float* dot1;
+  double* partials;
   int n1, n2, n3;
 };
 
 size_t fused_weights_bytes(int C);
+unsigned fused_blocks(int n1, int n2, int n3);
 bool fused_eligible(int C, int B, int n1, int n2, int n3);
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st);
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream_t st);

@@ -77,20 +77,41 @@ __device__ __forceinline__ TA upd(float x, float p, TA a, bool has_p) {
   return has_p ? (TA)(float)((TA)x + a * (TA)p) : (TA)x;
 }
 
+// Consecutive lanes of a wave own consecutive runs of a z-row (run index = wave base +
+// lane), so the z-halo of a run is the edge value of the neighbouring lane's centre row:
+// a DPP wave shift instead of a memory load.  wave_from_prev: lane l receives lane l-1's
+// value; wave_from_next: lane l+1's.  Lanes 0 / 63 receive garbage and load instead.
+__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float wave_from_prev(float v) { return __builtin_bit_cast(float, dpp_prev(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ float wave_from_next(float v) { return __builtin_bit_cast(float, dpp_next(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ double wave_from_prev(double v) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const long long r = ((long long)(unsigned)dpp_prev((int)(u >> 32)) << 32) | (unsigned)dpp_prev((int)u);
+  return __builtin_bit_cast(double, r);
+}
+__device__ __forceinline__ double wave_from_next(double v) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const long long r = ((long long)(unsigned)dpp_next((int)(u >> 32)) << 32) | (unsigned)dpp_next((int)u);
+  return __builtin_bit_cast(double, r);
+}
+
 template <int VZ, typename TA>
 __device__ __forceinline__ void load_star(const float* __restrict__ x, const float* __restrict__ p,
                                           TA a, const Run<VZ>& r, int n1, int n2, int n3,
                                           TA (&c)[VZ + 2], TA (&s)[4][VZ]) {
   const long s1 = (long)n2 * n3;
+  const int lane = threadIdx.x & 63;
   float xv[VZ], pv[VZ];
   load_vec<VZ>(x + r.v0, xv);
   if (p) load_vec<VZ>(p + r.v0, pv);
 #pragma unroll
   for (int t = 0; t < VZ; ++t) c[t + 1] = upd<TA>(xv[t], p ? pv[t] : 0.f, a, p != nullptr);
-  c[0] = TA(0);
-  c[VZ + 1] = TA(0);
-  if (r.k0 > 0) c[0] = upd<TA>(x[r.v0 - 1], p ? p[r.v0 - 1] : 0.f, a, p != nullptr);
-  if (r.k0 + VZ < n3) c[VZ + 1] = upd<TA>(x[r.v0 + VZ], p ? p[r.v0 + VZ] : 0.f, a, p != nullptr);
+  TA lft = wave_from_prev(c[VZ]), rgt = wave_from_next(c[1]);
+  if (lane == 0 && r.k0 > 0) lft = upd<TA>(x[r.v0 - 1], p ? p[r.v0 - 1] : 0.f, a, p != nullptr);
+  if (lane == 63 && r.k0 + VZ < n3) rgt = upd<TA>(x[r.v0 + VZ], p ? p[r.v0 + VZ] : 0.f, a, p != nullptr);
+  c[0] = r.k0 > 0 ? lft : TA(0);
+  c[VZ + 1] = r.k0 + VZ < n3 ? rgt : TA(0);
   const long offs[4] = {-s1, s1, -(long)n3, (long)n3};
   const bool ok[4] = {r.i > 0, r.i < n1 - 1, r.j > 0, r.j < n2 - 1};
 #pragma unroll
@@ -410,12 +431,16 @@ template <int VZ>
 __device__ __forceinline__ void load_dir_star(const DirIn& d, const Run<VZ>& r, int n1, int n2,
                                               int n3, float (&c)[VZ + 2], float (&s)[4][VZ]) {
   const long s1 = (long)n2 * n3;
+  const int lane = threadIdx.x & 63;
   float cv[VZ];
   dir_vec<VZ>(d, r.v0, cv);
 #pragma unroll
   for (int t = 0; t < VZ; ++t) c[t + 1] = cv[t];
-  c[0] = r.k0 > 0 ? dir_at(d, r.v0 - 1) : 0.f;
-  c[VZ + 1] = r.k0 + VZ < n3 ? dir_at(d, r.v0 + VZ) : 0.f;
+  float lft = wave_from_prev(c[VZ]), rgt = wave_from_next(c[1]);  // see load_star
+  if (lane == 0 && r.k0 > 0) lft = dir_at(d, r.v0 - 1);
+  if (lane == 63 && r.k0 + VZ < n3) rgt = dir_at(d, r.v0 + VZ);
+  c[0] = r.k0 > 0 ? lft : 0.f;
+  c[VZ + 1] = r.k0 + VZ < n3 ? rgt : 0.f;
   const long offs[4] = {-s1, s1, -(long)n3, (long)n3};
   const bool ok[4] = {r.i > 0, r.i < n1 - 1, r.j > 0, r.j < n2 - 1};
 #pragma unroll
@@ -499,7 +524,7 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_csr_kernel(CsrArgs a, const
 // Finalisation of the two-kernel reductions: one block sums the partials in block order
 // and updates the solver scalars.
 // ---------------------------------------------------------------------------
-enum FinOp : int { kFinResidInit = 0, kFinResid, kFinPW, kFinRZ, kFinDot, kFinProj, kFinNpsd };
+enum FinOp : int { kFinResidInit = 0, kFinResid, kFinPW, kFinRZ, kFinDot, kFinProj, kFinNpsd, kFinRZ2 };
 
 // aux/aux2: the output of kFinDot; alfa of N and O for kFinProj; alfa of the new
 // direction for kFinNpsd.
@@ -537,10 +562,12 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
       st->pw = t[0];
       st->alpha = st->rho / t[0];
       break;
-    case kFinRZ: {
+    case kFinRZ:
+    case kFinRZ2: {  // kFinRZ: t1 = z.(r - r_old); kFinRZ2: t1 = z.r_old
       const bool flex = (st->method == kNpcgFlex);
       st->rho = t[0];
-      st->beta = (st->iter == 0) ? 0.0 : (flex ? t[1] : t[0]) / st->rho_old;
+      const double num = flex ? (op == kFinRZ ? t[1] : t[0] - t[1]) : t[0];
+      st->beta = (st->iter == 0) ? 0.0 : num / st->rho_old;
       break;
     }
     case kFinDot:
@@ -659,6 +686,17 @@ int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, const float* aN, doub
   npsd_dir_csr_kernel<<<nb, kThreads, 0, st>>>(a, pN, aN);
   finalize(kFinNpsd, 3, a.partials, nb, a.st, alfa_out, st);
   return check_launch("npsd_dir_csr");
+}
+
+int launch_finalize_rz(double* partials, unsigned nb, SolveState* s, hipStream_t st) {
+  finalize(kFinRZ2, 2, partials, nb, s, nullptr, st);
+  return check_launch("finalize_rz");
+}
+
+int launch_finalize_proj(double* partials, unsigned nb, SolveState* s, const double* alfaN,
+                         const double* alfaO, hipStream_t st) {
+  finalize(kFinProj, 2, partials, nb, s, const_cast<double*>(alfaN), st, alfaO);
+  return check_launch("finalize_proj");
 }
 
 int launch_npsd_proj(const float* q, const float* aN, const float* aO, const double* alfaN,

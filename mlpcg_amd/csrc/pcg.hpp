@@ -74,6 +74,11 @@ int launch_p_spmv_csr(const CsrArgs& a, hipStream_t st);
 int launch_resid_csr(const CsrArgs& a, int init, hipStream_t st);
 int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, const float* aN, double* alfa_out,
                         hipStream_t st);
+// Finalizers for dot products computed elsewhere (the fused apply's epilogue):
+// partials [2][nb] of z.r and z.r_old (-> rho, beta), or of q.AN and q.AO (-> proj).
+int launch_finalize_rz(double* partials, unsigned nb, SolveState* s, hipStream_t st);
+int launch_finalize_proj(double* partials, unsigned nb, SolveState* s, const double* alfaN,
+                         const double* alfaO, hipStream_t st);
 int launch_npsd_proj(const float* q, const float* aN, const float* aO, const double* alfaN,
                      const double* alfaO, long n, SolveState* s, double* partials, hipStream_t st);
 int launch_spmv_csr(const int* rowptr, const int* col, const float* val, const float* x, float* y,

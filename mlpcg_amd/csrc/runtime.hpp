@@ -48,19 +48,26 @@ struct Precond {
 
   int update(const void* image, const void* const* weights, const void* const* biases,
              hipStream_t st);
+  // z = P(r * in_scale).  With dot0 (and optionally dot1) and partials, a plan whose last
+  // block is fused also writes per-block partial sums [2][*dot_blocks] of z.dot0, z.dot1
+  // (flat layout); *dot_blocks = 0 means they were not computed.
   int apply(const float* r, float* z, const double* in_scale, const uint8_t* out_mask,
-            const int* done, hipStream_t st);
+            const int* done, hipStream_t st, const float* dot0 = nullptr,
+            const float* dot1 = nullptr, double* partials = nullptr,
+            unsigned* dot_blocks = nullptr);
   int apply_block(int k, const void* x, int tx, void* y, int ty, const double* in_scale,
                   hipStream_t st);
   int run_block(int k, int level, const void* x, int tx, void* y, int ty, const void* prev,
                 const void* c0, const void* c1, const uint8_t* ma, const uint8_t* mb,
-                const double* sc, const int* done, hipStream_t st);
+                const double* sc, const int* done, hipStream_t st, const float* dot0 = nullptr,
+                const float* dot1 = nullptr, double* partials = nullptr);
   ~Precond();
 };
 
-// Plans created while this is set use the fused matrix-core apply for eligible 16-bit
-// blocks (process-wide; default on).
-void set_fused_apply(int enable);
+// Plans created while this is set use the fused matrix-core apply for eligible BF16
+// blocks: 0 never, 1 (default) on grids of at least 2^20 voxels, 2 on every eligible grid
+// (process-wide).
+void set_fused_apply(int mode);
 int fused_apply_enabled();
 
 int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3, int dtype,

@@ -3,7 +3,7 @@
 //
 // One iteration is a fixed sequence of kernels on the caller's stream; the scalars
 // (rho, beta, alpha, ||r||, the stop decision) live in a device SolveState updated by the
-// reductions' last blocks.  The host runs up to kLookahead iterations ahead of the GPU:
+// one-block finalize kernels of the reductions.  The host runs up to kLookahead iterations ahead of the GPU:
 // after queueing iteration k it waits only for the status of iteration k - kLookahead,
 // copied on a side stream.  Once the device sets `done`, the kernels of already-queued
 // iterations return immediately, so the iteration count and the result are exactly
@@ -193,13 +193,17 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
     TRY(launch_resid_csr(ca, 1, run));
   }
 
-  auto precondition = [&](const float* rin, float* zout, hipStream_t s) -> int {
+  // z = P(r); on the flat layout a fused last block also reduces z.dot0 and z.dot1
+  // (*nbd partial blocks, 0 if not)
+  auto precondition = [&](const float* rin, float* zout, hipStream_t s, const float* dot0 = nullptr,
+                          const float* dot1 = nullptr, unsigned* nbd = nullptr) -> int {
+    if (nbd) *nbd = 0;
     if (!P) {
       MLPCG_HIP(hipMemcpyAsync(zout, rin, vb, hipMemcpyDeviceToDevice, s));
       return 0;
     }
     const double* scale = &this->st->scale;
-    if (layout == 0) return P->apply(rin, zout, scale, code, done, s);
+    if (layout == 0) return P->apply(rin, zout, scale, code, done, s, dot0, dot1, partials, nbd);
     int rc2 = launch_scatter(fluid_idx, rin, grid_in, n, done, s);
     if (rc2) return rc2;
     rc2 = P->apply(grid_in, grid_out, scale, nullptr, done, s);
@@ -220,8 +224,12 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
     if (rc2) return rc2; \
   } while (0)
     if (method != kNpsd) {
-      TRY2(precondition(rk, z, s));
-      TRY2(launch_rz_dot(rk, z, r[nxt], n, this->st, partials, counter, s));
+      unsigned nbd = 0;
+      TRY2(precondition(rk, z, s, rk, method == kNpcgFlex ? r[nxt] : nullptr, &nbd));
+      if (nbd)
+        TRY2(launch_finalize_rz(partials, nbd, this->st, s));
+      else
+        TRY2(launch_rz_dot(rk, z, r[nxt], n, this->st, partials, counter, s));
       if (layout == 0) {
         fa.z = z;
         fa.p_old = p[cur];
@@ -250,16 +258,21 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
     // npsd: q = P(r), projected against the kept directions N (newest) and O; direction
     // d lives in q_hist[d % 3], its A-product and alfa in slot d % 2.
     float* q = z;
-    TRY2(precondition(rk, q, s));
     const int kept = cnt < 2 ? cnt : 2;
     const float* pN = kept >= 1 ? q_hist[(cnt - 1) % 3] : nullptr;
     const float* pO = kept >= 2 ? q_hist[(cnt - 2) % 3] : nullptr;
     const float* aN = kept >= 1 ? aq_hist[(cnt - 1) & 1] : nullptr;
-    if (kept >= 1)
-      TRY2(launch_npsd_proj(q, aN, kept >= 2 ? aq_hist[cnt & 1] : nullptr,
-                            &this->st->alfa[(cnt - 1) & 1],
-                            kept >= 2 ? &this->st->alfa[cnt & 1] : nullptr, n, this->st, partials,
-                            s));
+    const float* aO = kept >= 2 ? aq_hist[cnt & 1] : nullptr;
+    const double* alfaN = &this->st->alfa[(cnt - 1) & 1];
+    const double* alfaO = kept >= 2 ? &this->st->alfa[cnt & 1] : nullptr;
+    unsigned nbd = 0;
+    TRY2(precondition(rk, q, s, aN, aO, &nbd));
+    if (kept >= 1) {
+      if (nbd)
+        TRY2(launch_finalize_proj(partials, nbd, this->st, alfaN, alfaO, s));
+      else
+        TRY2(launch_npsd_proj(q, aN, aO, alfaN, alfaO, n, this->st, partials, s));
+    }
     float* q_new = q_hist[cnt % 3];
     if (layout == 0) {
       fa.z = q;

@@ -17,8 +17,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _plan(m, img, dtype, fused):
+    """fused: 2 forces the fused kernel on every eligible grid (small test grids), 0 off."""
     from mlpcg_amd.precond import set_fused_apply
-    prev = set_fused_apply(fused)
+    prev = set_fused_apply(2 if fused else 0)
     try:
         return m.make_preconditioner(img, dtype)
     finally:
@@ -143,3 +144,27 @@ def test_fused_bf16_solve_iterations(dev):
         assert np.linalg.norm(b - A @ xs) / np.linalg.norm(b) < 1.05e-6
         its[name] = info["iters"]
     assert its["bf16"] <= 1.1 * its["fp32"] + 1, its
+
+
+@pytest.mark.parametrize("method", ["npcg", "npcg_flex", "npsd"])
+def test_fused_epilogue_dots_in_solver(dev, method):
+    """With a fused last block the solver's dot products on z (r.z, r_old.z; q.AN, q.AO)
+    come from the apply's epilogue instead of separate kernels.  With exactly representable
+    stencils both plans give the same z up to FP32 summation order, so the residual
+    histories must agree; a wrong pairing of dots would separate them at once."""
+    from mlpcg_amd.solver import FusedPCG
+    N = 64
+    sc = S.scene_spheres(N, 11, num_spheres=3, rmin=4, rmax=9)
+    b = S.make_rhs(sc.flags, 2)[0]
+    m = _exact_model(2, 1, False, seed=9)
+    img = _exact_image(1, (N, N, N), seed=4, n=2)
+    s = FusedPCG(sc.flags, "flags", device=dev)
+    bv = s.vector(b)
+    hist = {}
+    for fused in (True, False):
+        plan = _plan(m, img, torch.bfloat16, fused)
+        _, info = s.solve(bv, plan, method=method, max_it=12, tol=1e-12, history=True)
+        assert info["iters"] == 12
+        hist[fused] = info["history"]
+    rel = np.abs(hist[True] - hist[False]) / hist[False]
+    assert rel.max() < 1e-3, rel

@@ -333,7 +333,8 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the preconditioned solve of one scene is a "
-              "single-GPU workload here (see DESIGN.md for how it would shard)", file=sys.stderr)
+              "single-GPU workload here (its z-slab decomposition over GPUs is a design, not "
+              "a code path)", file=sys.stderr)
         sys.exit(2)
     import torch
     from mlpcg_amd import scenes as S

@@ -149,22 +149,23 @@ __device__ __forceinline__ void block_partials(const double (&v)[NR], double* __
 }
 
 // Sum of partials[r * nb + 0 .. nb) by one block of kFinThreads threads, in a fixed
-// order (per-thread strided sums with 4 independent accumulators, then a tree over the
+// order (per-thread strided sums with 8 independent accumulators, then a tree over the
 // threads); the result is valid in thread 0.
 constexpr int kFinThreads = 1024;
 
 __device__ __forceinline__ double sum_partials(const double* __restrict__ p, unsigned nb) {
   __shared__ double sh[kFinThreads / 64];
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  // up to 8 independent loads per thread in flight (nb <= 8 * kFinThreads in one round)
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   unsigned b = threadIdx.x;
-  for (; b + 3 * kFinThreads < nb; b += 4 * kFinThreads) {
-    a0 += p[b];
-    a1 += p[b + kFinThreads];
-    a2 += p[b + 2 * kFinThreads];
-    a3 += p[b + 3 * kFinThreads];
+  for (; b + 7 * kFinThreads < nb; b += 8 * kFinThreads) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += p[b + u * kFinThreads];
   }
-  for (; b < nb; b += kFinThreads) a0 += p[b];
-  double s = wave_sum((a0 + a1) + (a2 + a3));
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if (b + u * kFinThreads < nb) a[u] += p[b + u * kFinThreads];
+  double s = wave_sum(((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();

@@ -533,6 +533,26 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
                                                                unsigned nb, SolveState* st,
                                                                double* aux, const double* aux2) {
   if (op != kFinResidInit && op != kFinDot && st->done) return;  // uniform over the block
+  // thread 0 fetches the scalars it will read while the block sums the partials
+  double rho = 0.0, rho_old = 0.0, norm_b = 0.0, tol = 0.0, atol = 0.0, cross = 0.0, an = 0.0, ao = 0.0;
+  int iter = 0, max_it = 0, method = 0;
+  double* hist = nullptr;
+  if (threadIdx.x == 0 && st) {
+    rho = st->rho;
+    rho_old = st->rho_old;
+    norm_b = st->norm_b;
+    tol = st->tol;
+    atol = st->atol;
+    cross = st->cross;
+    iter = st->iter;
+    max_it = st->max_it;
+    method = st->method;
+    hist = st->history;
+    if (op == kFinProj) {
+      an = aux[0];
+      ao = aux2 ? aux2[0] : 1.0;
+    }
+  }
   double t[3] = {0.0, 0.0, 0.0};
   for (int r = 0; r < nr; ++r) t[r] = sum_partials(partials + (size_t)r * nb, nb);
   if (threadIdx.x != 0) return;
@@ -540,43 +560,46 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
     case kFinResidInit:
     case kFinResid: {
       if (op == kFinResidInit) {
-        st->norm_b = sqrt(t[1]);
-        st->iter = 0;
+        norm_b = sqrt(t[1]);
+        st->norm_b = norm_b;
+        iter = 0;
         st->done = (t[0] == 0.0) ? 1 : 0;
       } else {
-        st->iter += 1;
-        st->rho_old = st->rho;
+        iter += 1;
+        st->rho_old = rho;
       }
+      st->iter = iter;
       st->rr = t[0];
-      st->norm = sqrt(t[0]) / st->norm_b;
+      const double norm = sqrt(t[0]) / norm_b;
+      st->norm = norm;
       st->scale = t[0] > 0.0 ? 1.0 / sqrt(t[0]) : 0.0;
       if (op == kFinResid) {
-        const double tol = fmax(st->tol, st->atol / st->norm_b);
-        const bool cont = (st->method == kNpsd) ? !(st->norm < tol) : (st->norm > tol);
-        st->done = (cont && st->iter < st->max_it) ? 0 : 1;
+        const double stop = fmax(tol, atol / norm_b);
+        const bool cont = (method == kNpsd) ? !(norm < stop) : (norm > stop);
+        st->done = (cont && iter < max_it) ? 0 : 1;
       }
-      if (st->history && st->iter <= st->max_it) st->history[st->iter] = st->norm;
+      if (hist && iter <= max_it) hist[iter] = norm;
       break;
     }
     case kFinPW:
       st->pw = t[0];
-      st->alpha = st->rho / t[0];
+      st->alpha = rho / t[0];
       break;
     case kFinRZ:
     case kFinRZ2: {  // kFinRZ: t1 = z.(r - r_old); kFinRZ2: t1 = z.r_old
-      const bool flex = (st->method == kNpcgFlex);
+      const bool flex = (method == kNpcgFlex);
       st->rho = t[0];
       const double num = flex ? (op == kFinRZ ? t[1] : t[0] - t[1]) : t[0];
-      st->beta = (st->iter == 0) ? 0.0 : num / st->rho_old;
+      st->beta = (iter == 0) ? 0.0 : num / rho_old;
       break;
     }
     case kFinDot:
       aux[0] = t[0];
       break;
     case kFinProj: {
-      const double gN = t[0] / aux[0];
+      const double gN = t[0] / an;
       st->proj[0] = gN;
-      st->proj[1] = aux2 ? (t[1] - gN * st->cross) / aux2[0] : 0.0;
+      st->proj[1] = aux2 ? (t[1] - gN * cross) / ao : 0.0;
       break;
     }
     case kFinNpsd:

@@ -176,22 +176,33 @@ def copy_bandwidth(dev, nbytes=2 << 30):
     return 2 * nbytes / t / 1e9
 
 
-def train_rate(dev, steps=3, warmup=1, n=128, scenes=8, levels=5):
+def train_rate(dev, steps=3, warmup=1, n=128, scenes=8, levels=5, graphed_steps=20):
     """Adam steps/s on 8 seeded 128^3 scenes (one N(0,1) RHS each), loss
-    ||b - A P(b)||^2/||b||^2, hand-written backward kernels."""
+    ||b - A P(b)||^2/||b||^2, hand-written backward kernels: eager steps, and the same step
+    captured in a HIP graph (fresh right-hand sides copied in before every replay)."""
     import torch
     from mlpcg_amd import train as T
     from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
-    torch.manual_seed(1)
-    model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
-    opt = torch.optim.Adam(model.parameters(), lr=1e-4)
     batches = [T.SceneBatch(s, 1, dev) for s in T.make_scenes(n, range(200, 200 + scenes))]
     gen = torch.Generator(device=dev).manual_seed(2)
     rhs = [T.white_rhs(sb, 1, gen) for sb in batches]
+    torch.manual_seed(1)
+    model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
+    opt = torch.optim.Adam(model.parameters(), lr=1e-4)
     for _ in range(warmup):
         T.train_step(model, batches, rhs, opt)
     t, _ = timed(lambda: [T.train_step(model, batches, rhs, opt) for _ in range(steps)])
-    return steps / t, torch.cuda.max_memory_allocated(dev)
+    eager = steps / t
+    torch.manual_seed(1)
+    model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
+    opt = torch.optim.Adam(model.parameters(), lr=1e-4, capturable=True)
+    step = T.GraphedTrainStep(model, batches, rhs, opt)
+    pool = [[T.white_rhs(sb, 1, gen) for sb in batches] for _ in range(4)]
+    step(pool[0])
+    t, _ = timed(lambda: [step(pool[k % 4]) for k in range(graphed_steps)])
+    loss = float(step.loss)
+    return {"steps_per_s": round(graphed_steps / t, 3), "eager_steps_per_s": round(eager, 3),
+            "last_loss": loss, "peak_hbm_GiB": round(torch.cuda.max_memory_allocated(dev) / 2 ** 30, 2)}
 
 
 def cpu_baseline(flags, image_np, model, iters, levels):
@@ -460,10 +471,10 @@ def main():
 
     train = None
     if not args.skip_train:
-        sps, peak = train_rate(dev)
-        train = {"steps_per_s": round(sps, 3), "workload": "8 scenes x 128^3, 1 RHS each, l=5, FP32, Adam",
-                 "peak_hbm_GiB": round(peak / 2 ** 30, 2)}
-        log(f"[bench] train: {sps:.2f} steps/s")
+        train = train_rate(dev)
+        train["workload"] = "8 scenes x 128^3, 1 RHS each, l=5, FP32, Adam; step captured in a HIP graph"
+        log(f"[bench] train: {train['steps_per_s']:.2f} steps/s graphed, "
+            f"{train['eager_steps_per_s']:.2f} eager")
 
     cpu = None
     if not args.skip_cpu:

@@ -177,6 +177,12 @@ void mlpcg_precond_destroy(mlpcg_precond* P);
  * block); 1 (the default): on such grids of at least 2^20 voxels (smaller ones are faster
  * with stored K); 2: on every such grid.  Process-wide; returns the previous mode. */
 int mlpcg_set_fused_apply(int mode);
+/* No counterpart in the reference: SmallSM plans (swap off, FP32/FP16/BF16) created after
+ * this call run their two coarsest levels (at most 4096 voxels on level n-1) as one
+ * workgroup kernel with the level vectors in LDS (enable = 1) instead of seven small
+ * kernels (enable = 0, the default: at 256^3, l=5 the single workgroup takes 85 us
+ * against 45 us for the seven kernels).  Process-wide; returns the previous setting. */
+int mlpcg_set_coarse_tail(int enable);
 /* No counterpart in the reference (it has no plan object): y = block(x * (*in_scale)) for
  * stencil block `block` of the plan alone (block order of the weights array above; not a
  * transposed one), on the grid of its level, without level combine or masks; x and y of

@@ -316,6 +316,12 @@ int mlpcg_precond_apply_block(mlpcg_precond* P, int block, const void* x, int x_
   return P->impl->apply_block(block, x, x_dtype, y, y_dtype, in_scale, STREAM(stream));
 }
 
+int mlpcg_set_coarse_tail(int enable) {
+  const int prev = coarse_tail_enabled();
+  set_coarse_tail(enable);
+  return prev;
+}
+
 int mlpcg_set_fused_apply(int mode) {
   const int prev = fused_apply_enabled();
   set_fused_apply(mode);

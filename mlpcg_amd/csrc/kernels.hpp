@@ -67,6 +67,24 @@ bool fused_eligible(int C, int B, int n1, int n2, int n3);
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st);
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream_t st);
 
+// The two coarsest levels of the SmallSM V-cycle in one workgroup (tail.hip): from the
+// output x_in of pre[n-2] (level n-2 grid) to up(c0 x + c1 post[n-1](up(l(pool(x))))) on
+// the level n-2 grid.  f1..f3: level n-1 dims (at most kTailMaxFine voxels, even).
+constexpr int kTailMaxFine = 4096;
+struct TailArgs {
+  const void* x_in;
+  const void* K_pre;
+  const void* K_post;
+  const void* K_l;
+  const float* c0;
+  const float* c1;
+  void* out;
+  int nearest;
+  int f1, f2, f3;
+};
+bool tail_eligible(int dtype, int f1, int f2, int f3);
+int launch_coarse_tail(int dtype, const TailArgs& a, hipStream_t st);
+
 int launch_stencil_weights(int dtype, const void* img, const void* W, const void* b, void* K, int C,
                            int n1, int n2, int n3, bool trans, hipStream_t st);
 int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& a, hipStream_t st);

@@ -30,6 +30,11 @@ static int g_fused_apply = 1;
 constexpr long kFusedMinVoxels = 1L << 20;
 void set_fused_apply(int mode) { g_fused_apply = (mode < 0 || mode > 2) ? 1 : mode; }
 int fused_apply_enabled() { return g_fused_apply; }
+// off by default: one workgroup streams the coarse stencils too slowly (85 us at 256^3,
+// l=5, against 45 us for the seven per-level kernels it replaces)
+static int g_coarse_tail = 0;
+void set_coarse_tail(int enable) { g_coarse_tail = enable ? 1 : 0; }
+int coarse_tail_enabled() { return g_coarse_tail; }
 
 Precond::~Precond() {
   for (void* p : allocs) (void)hipFree(p);
@@ -128,6 +133,12 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
     P->K.push_back(K);
     P->wpack.push_back(wp);
   }
+  // the two coarsest SmallSM levels in one workgroup kernel when they are small enough
+  P->tail = (g_coarse_tail && kind == kSmallSM && !swap && n >= 2 &&
+             tail_eligible(dtype, P->dims[n - 1][0], P->dims[n - 1][1], P->dims[n - 1][2]) &&
+             !P->blocks[n - 1].fused && !P->blocks[2 * n - 1].fused && !P->blocks[2 * n].fused)
+                ? 1
+                : 0;
   TRY(palloc(P, &P->coef, (size_t)4 * n * as));
   TRY(palloc(P, &P->lin_ws, mean_patch_workspace(P->acc, C, n1, n2, n3)));
   TRY(palloc(P, &P->ybar, (size_t)27 * C * as));
@@ -282,17 +293,34 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
   // ---- downward half (identical for every kind): x_0 = B0(r), x_i = Bi(pool x_{i-1}) ----
   TRY(apply_blk(0, 0, r_in, vdt, xd[0], dtype, nullptr, nullptr, nullptr, mask ? maskimg : nullptr,
                 nullptr, in_scale));
-  for (int i = 1; i < n; ++i) {
+  const int ndown = tail ? n - 1 : n;  // with the tail, levels n-1 and n run in one kernel
+  for (int i = 1; i < ndown; ++i) {
     TRY(pool(xd[i - 1], tmp[i], i - 1));
     TRY(apply_blk(i, i, tmp[i], dtype, xd[i], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
   }
-  TRY(pool(xd[n - 1], tmp[n], n - 1));
   const int kl = (kind == kSmallSPD) ? n : 2 * n;  // index of l
-  TRY(apply_blk(kl, n, tmp[n], dtype, xd[n], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+  if (tail) {
+    TailArgs t;
+    t.x_in = xd[n - 2];
+    t.K_pre = K[n - 1];
+    t.K_post = K[2 * n - 1];
+    t.K_l = K[2 * n];
+    t.c0 = (const float*)c0p(n - 1, 0);
+    t.c1 = (const float*)c1p(n - 1, 0);
+    t.out = tmp[n - 2];
+    t.nearest = nearest;
+    t.f1 = dims[n - 1][0];
+    t.f2 = dims[n - 1][1];
+    t.f3 = dims[n - 1][2];
+    TRY(launch_coarse_tail(dtype, t, st));
+  } else {
+    TRY(pool(xd[n - 1], tmp[n], n - 1));
+    TRY(apply_blk(kl, n, tmp[n], dtype, xd[n], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+  }
 
   if (kind == kSmallSM) {
     const void* cur = xd[n];
-    for (int i = n; i >= 1; --i) {
+    for (int i = tail ? n - 1 : n; i >= 1; --i) {
       const int lvl = i - 1;
       const bool last = (lvl == 0);
       void* y = last ? (void*)z_out : xo[lvl];
@@ -300,7 +328,7 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
       const uint8_t* ma = (last && mask) ? maskimg : nullptr;
       const uint8_t* mb = last ? out_mask : nullptr;
       if (!swap) {
-        TRY(up(cur, tmp[lvl], lvl, 1.0, nullptr, nullptr, nullptr));
+        if (!(tail && i == n - 1)) TRY(up(cur, tmp[lvl], lvl, 1.0, nullptr, nullptr, nullptr));
         if (last && dot0 && partials && dot_blocks && blocks[n].fused) {
           // the solver's dot products on z ride in the epilogue of the last block
           TRY(run_block(n, 0, tmp[0], dtype, y, ty, xd[0], c0p(0, 0), c1p(0, 0), ma, mb, nullptr, done, st,

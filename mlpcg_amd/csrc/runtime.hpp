@@ -40,6 +40,7 @@ struct Precond {
   std::vector<Blk> blocks;          // one per entry of K (nullptr for fused blocks)
   std::vector<void*> wpack;         // fused blocks: packed MFMA weights + bias init
   void* imgw[kMaxLevels + 1] = {};  // FP16 image of levels with fused blocks
+  int tail = 0;                     // SmallSM: levels n-1 and n in one workgroup kernel
   int nw = 0;                       // number of block weights before c0/c1
   void* lin_ws = nullptr;           // SMLinear reduction workspace
   void* ybar = nullptr;             // SMLinear mean patch scratch
@@ -69,6 +70,11 @@ struct Precond {
 // (process-wide).
 void set_fused_apply(int mode);
 int fused_apply_enabled();
+
+// Plans created while this is set run the two coarsest SmallSM levels in one workgroup
+// kernel when they fit (default off).
+void set_coarse_tail(int enable);
+int coarse_tail_enabled();
 
 int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3, int dtype,
                    int nearest, int mask, int swap, const void* image, const void* const* weights,

@@ -29,6 +29,13 @@ def set_fused_apply(mode) -> int:
     return int(lib.mlpcg_set_fused_apply(int(mode)))
 
 
+def set_coarse_tail(enable) -> bool:
+    """Process-wide switch (default off) for SmallSM plans created afterwards: the two
+    coarsest levels run as one workgroup kernel when level n-1 has at most 4096 voxels.
+    Returns the previous setting."""
+    return bool(lib.mlpcg_set_coarse_tail(int(bool(enable))))
+
+
 class Preconditioner:
     """z = P(r) for a fixed image.  dtype: storage type of K and of the level vectors
     (torch.float32 / bfloat16 / float16; float64 for checking)."""

@@ -316,6 +316,46 @@ class SmallSMModelDn3D(BaseModel):
                 x[i - 1] = _comb(c0, x[i - 1], c1, x[i])
         return x[0]
 
+    # -- training with a fixed scene image ------------------------------------------
+    def image_cache(self, image, dtype=None):
+        """The parts of forward() that depend on the image only: the pyramid (in `dtype`,
+        the activation type; forward casts the image to it) and the SMLinear mean patches
+        ybar of levels 0..n-1 (sm_linear_3d_kernel.cu:62-83).  A training loop that reuses
+        a scene image computes them once and calls forward_cached."""
+        with torch.no_grad():
+            imgs = [image.to(dtype or image.dtype).contiguous()]
+            for _ in range(self.n):
+                imgs.append(K.avgpool2(imgs[-1]))
+            ybars = [K.sm_linear(imgs[i], self.c0[i].weight, self.c0[i].bias)[1]
+                     for i in range(self.n)]
+        return imgs, ybars
+
+    def forward_cached(self, cache, b):
+        """forward(image, b) with the image pyramid and mean patches of image_cache: same
+        result (c = w . ybar + bias by torch, whose gradients are smlinear3d.backward's)."""
+        imgs, ybars = cache
+
+        def lin(m, ybar):
+            return (m.weight.reshape(-1).to(ybar.dtype) * ybar.reshape(-1)).sum().reshape(1) + m.bias.to(ybar.dtype)
+
+        x = [self.pre[0](imgs[0], b)]
+        for i in range(1, self.n):
+            x.append(avg_pool(x[-1]))
+            x[-1] = self.pre[i](imgs[i], x[-1])
+        x.append(avg_pool(x[-1]))
+        x[-1] = self.l(imgs[self.n], x[-1])
+        for i in range(self.n, 0, -1):
+            x[i] = upsample(x[i], self.mode)
+            c0 = lin(self.c0[i - 1], ybars[i - 1])
+            c1 = lin(self.c1[i - 1], ybars[i - 1])
+            if self.swap_ord:
+                x[i - 1] = _comb(c0, x[i - 1], c1, x[i])
+                x[i - 1] = self.post[i - 1](imgs[i - 1], x[i - 1])
+            else:
+                x[i] = self.post[i - 1](imgs[i - 1], x[i])
+                x[i - 1] = _comb(c0, x[i - 1], c1, x[i])
+        return x[0]
+
     # -- native plan ----------------------------------------------------------------
     def plan_params(self):
         blocks = list(self.pre) + list(self.post) + [self.l]

@@ -58,6 +58,15 @@ class SceneBatch:
         self.image = torch.as_tensor(S.image_channels(scene.flags, num_imgs), device=device).to(image_dtype)
         lab, closed = S.fluid_components_without_air(scene.flags)
         self.closed = [torch.as_tensor((lab == c).ravel(), device=device) for c in closed]
+        self._caches = {}
+
+    def model_input(self, model, dtype):
+        """What model.forward_cached needs for this scene (pyramid, mean patches), built on
+        first use per (levels, activation type)."""
+        key = (model.n, dtype)
+        if key not in self._caches:
+            self._caches[key] = model.image_cache(self.image, dtype)
+        return self._caches[key]
 
 
 def white_rhs(sb: SceneBatch, count: int, gen: torch.Generator):
@@ -104,6 +113,40 @@ def relative_residual_sq(sb: SceneBatch, x, b):
     return ((r * r).sum(1) / (b * b).sum(1)).mean()
 
 
+class RelResidualSqFunction(torch.autograd.Function):
+    """mean_b ||b_b - M A M x_b||^2 / ||b_b||^2 (M: restriction to the fluid cells) with the
+    reductions done by the library's two-kernel FP64 dot instead of torch reductions, so a
+    training step built on it can be captured in a HIP graph.  A is symmetric, hence
+    d/dx_b = -2 / (B ||b_b||^2) M A M r_b."""
+
+    @staticmethod
+    def forward(ctx, x, b, code, fluid):
+        B, V = b.shape
+        xf = (x.reshape(B, V).float() * fluid).contiguous()
+        b = b.contiguous()
+        r = torch.stack([bi - K.spmv_flags(code, xi) for bi, xi in zip(b, xf)])
+        nr = torch.stack([K.dot(ri, ri) for ri in r])
+        nb = torch.stack([K.dot(bi, bi) for bi in b])
+        loss = (nr / nb).sum()  # B entries: one workgroup, no partials to clear
+        ctx.save_for_backward(r, nb, code, fluid)
+        ctx.xshape, ctx.xdtype = x.shape, x.dtype
+        return (loss / B).float()
+
+    @staticmethod
+    def backward(ctx, g):
+        r, nb, code, fluid = ctx.saved_tensors
+        B = r.shape[0]
+        coef = (-2.0 / B) * g.double() / nb  # (B,)
+        gx = torch.stack([(c.float() * K.spmv_flags(code, (ri * fluid).contiguous())) * fluid
+                          for c, ri in zip(coef, r)])
+        return gx.reshape(ctx.xshape).to(ctx.xdtype), None, None, None
+
+
+def relative_residual_sq_dot(sb: SceneBatch, x, b):
+    """relative_residual_sq through RelResidualSqFunction (library reductions)."""
+    return RelResidualSqFunction.apply(x, b, sb.code, sb.fluid.reshape(1, -1).float())
+
+
 def residual_loss(sb: SceneBatch, x, b):
     """loss_functions.py:18-23: mean_b ||b - A x||."""
     B, V = b.shape
@@ -112,17 +155,75 @@ def residual_loss(sb: SceneBatch, x, b):
     return r.norm(dim=1).mean()
 
 
+def _step_body(model, batches, rhs, act_dtype, loss_fn):
+    """Forward + backward over the scenes; returns the mean loss as a device tensor."""
+    total = None
+    for sb, b in zip(batches, rhs):
+        bb = b.reshape(b.shape[0], 1, *sb.shape).to(act_dtype)
+        if hasattr(model, "forward_cached"):  # the scene image is fixed: reuse its pyramid
+            x = model.forward_cached(sb.model_input(model, act_dtype), bb)
+        else:
+            x = model(sb.image.to(act_dtype), bb)
+        loss = loss_fn(sb, x, b) / len(batches)
+        loss.backward()
+        total = loss.detach() if total is None else total + loss.detach()
+    return total
+
+
 def train_step(model, batches, rhs, optimizer, act_dtype=torch.float32, loss_fn=relative_residual_sq):
     """One Adam step over a list of (SceneBatch, rhs[B, V]) pairs; returns the mean loss."""
     optimizer.zero_grad(set_to_none=True)
-    total = 0.0
-    for sb, b in zip(batches, rhs):
-        x = model(sb.image.to(act_dtype), b.reshape(b.shape[0], 1, *sb.shape).to(act_dtype))
-        loss = loss_fn(sb, x, b) / len(batches)
-        loss.backward()
-        total += float(loss.detach())
+    total = _step_body(model, batches, rhs, act_dtype, loss_fn)
     optimizer.step()
-    return total
+    return float(total)
+
+
+class GraphedTrainStep:
+    """train_step captured once as a HIP graph and replayed: each call copies the
+    right-hand sides into static buffers and replays forward, backward and the Adam update;
+    the loss stays on the device.  The optimizer must be built with capturable=True (Adam
+    then keeps its step count on the device).  The loss uses the library's reductions
+    (relative_residual_sq_dot), so no torch reduction over many workgroups is recorded.
+    The warm-up steps that precede the capture run on copies of `rhs` and are undone
+    afterwards (parameters restored, optimizer moments and step counts cleared), so the
+    first replay is the optimizer's first step."""
+
+    def __init__(self, model, batches, rhs, optimizer, act_dtype=torch.float32,
+                 loss_fn=relative_residual_sq_dot, warmup=2):
+        dev = batches[0].code.device
+        self.rhs = [b.detach().clone() for b in rhs]
+        for sb in batches:  # image pyramids are built outside the graph
+            if hasattr(model, "forward_cached"):
+                sb.model_input(model, act_dtype)
+        params = [p for p in model.parameters()]
+        saved = [p.detach().clone() for p in params]
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                optimizer.zero_grad(set_to_none=False)
+                _step_body(model, batches, self.rhs, act_dtype, loss_fn)
+                optimizer.step()
+        torch.cuda.current_stream(dev).wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            optimizer.zero_grad(set_to_none=False)
+            self.loss = _step_body(model, batches, self.rhs, act_dtype, loss_fn)
+            optimizer.step()
+        with torch.no_grad():  # capture ran nothing: undo the warm-up steps
+            for p, s0 in zip(params, saved):
+                p.copy_(s0)
+            for state in optimizer.state.values():
+                for v in state.values():
+                    if torch.is_tensor(v):
+                        v.zero_()
+
+    def __call__(self, rhs):
+        """One Adam step on the given right-hand sides; returns the (device) mean loss."""
+        for dst, src in zip(self.rhs, rhs):
+            dst.copy_(src)
+        self.graph.replay()
+        return self.loss
 
 
 def make_scenes(n, seeds, kind="spheres"):

@@ -131,3 +131,46 @@ def test_plan_matches_module_eval_forward_f32(dev):
         ye = m.eval_forward(img.to(dev), r.to(dev), None, [], [], [])
     z = m.make_preconditioner(img.to(dev), torch.float32).apply(r.to(dev).reshape(shape))
     assert _rel(z.reshape(ye.shape), ye) < 1e-6
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 2e-6), (torch.bfloat16, 2e-3), (torch.float16, 5e-4)])
+@pytest.mark.parametrize("n,shape,mode", [(3, (32, 32, 32), "trilinear"), (2, (16, 32, 16), "nearest"),
+                                          (4, (64, 64, 64), "trilinear")])
+def test_coarse_tail_matches_kernel_sequence(dev, dtype, tol, n, shape, mode):
+    """The one-workgroup kernel for the two coarsest levels computes the same V-cycle tail
+    as the seven per-level kernels (same roundings to the storage type; FP32 summation
+    order may differ)."""
+    from mlpcg_amd.precond import set_coarse_tail
+    m, _ = _small_model(n, 1, mode, seed=31, dtype=torch.float32)
+    img, r = _image(1, shape, 32).float().to(dev), _rand((1, 1, *shape), 33).float().to(dev)
+    z = {}
+    for tail in (True, False):
+        prev = set_coarse_tail(tail)
+        try:
+            plan = m.make_preconditioner(img, dtype)
+        finally:
+            set_coarse_tail(prev)
+        z[tail] = plan.apply(r.reshape(shape)).double()
+    assert float((z[True] - z[False]).norm() / z[False].norm()) < tol
+
+
+@pytest.mark.parametrize("act", [torch.float32, torch.bfloat16])
+def test_forward_cached_matches_forward(dev, act):
+    """forward_cached (image pyramid and SMLinear mean patches computed once per scene)
+    gives forward's output and parameter gradients."""
+    n, C, shape = 3, 1, (32, 32, 32)
+    m, _ = _small_model(n, C, seed=41, dtype=torch.float32)
+    img = _image(C, shape, 42).float().to(dev)
+    b = _rand((2, 1, *shape), 43).float().to(dev).to(act)
+    R = _rand((2, 1, *shape), 44).float().to(dev)
+    outs, grads = [], []
+    for cached in (False, True):
+        m.zero_grad(set_to_none=True)
+        y = m.forward_cached(m.image_cache(img, act), b) if cached else m(img.to(act), b)
+        (y.float() * R).sum().backward()
+        outs.append(y.detach().float())
+        grads.append({k: p.grad.detach().clone() for k, p in m.named_parameters()})
+    tol = 1e-6 if act == torch.float32 else 1e-2
+    assert _rel(outs[1], outs[0]) < tol
+    for k in grads[0]:
+        assert _rel(grads[1][k], grads[0][k]) < (1e-5 if act == torch.float32 else 5e-2), k

@@ -1,0 +1,75 @@
+"""Training step captured as a HIP graph: the library-reduction loss against the torch
+formula, and graph replays against eager steps (every replay checked, with eager work and
+host synchronisations between the replays)."""
+import copy
+
+import pytest
+import torch
+
+from mlpcg_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(dev, n=32, levels=3, scenes=2, B=2):
+    from mlpcg_amd import train as T
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    torch.manual_seed(3)
+    model = SmallSMModelDn3D(levels, 1).to(dev)
+    sbs = [T.SceneBatch(S.scene_spheres(n, 10 + s, num_spheres=3, rmin=3, rmax=6), 1, dev)
+           for s in range(scenes)]
+    gen = torch.Generator(device=dev).manual_seed(4)
+    return T, model, sbs, gen, B
+
+
+def test_dot_loss_matches_torch_loss_and_gradient(dev):
+    T, model, sbs, gen, B = _setup(dev)
+    sb = sbs[0]
+    b = T.white_rhs(sb, B, gen)
+    x0 = model(sb.image, b.reshape(B, 1, *sb.shape)).detach()
+    vals, grads = [], []
+    for fn in (T.relative_residual_sq, T.relative_residual_sq_dot):
+        x = x0.clone().requires_grad_()
+        loss = fn(sb, x, b)
+        loss.backward()
+        vals.append(float(loss.detach()))
+        grads.append(x.grad.detach().clone())
+    assert abs(vals[0] - vals[1]) <= 1e-5 * max(1.0, abs(vals[0]))
+    assert float((grads[0] - grads[1]).norm() / grads[0].norm()) < 1e-5
+
+
+@pytest.mark.parametrize("act", [torch.float32, torch.bfloat16])
+def test_graphed_step_matches_eager_steps(dev, act):
+    T, model, sbs, gen, B = _setup(dev)
+    model_g = copy.deepcopy(model)
+    opt_e = torch.optim.Adam(model.parameters(), lr=3e-3, capturable=True)
+    opt_g = torch.optim.Adam(model_g.parameters(), lr=3e-3, capturable=True)
+    steps = [[T.white_rhs(sb, B, gen) for sb in sbs] for _ in range(4)]
+    graphed = T.GraphedTrainStep(model_g, sbs, steps[0], opt_g, act_dtype=act)
+    for p, q in zip(model.parameters(), model_g.parameters()):
+        assert torch.equal(p, q)  # warm-up undone
+    for k, rhs in enumerate(steps):
+        le = T.train_step(model, sbs, rhs, opt_e, act_dtype=act, loss_fn=T.relative_residual_sq_dot)
+        lg = float(graphed(rhs))
+        torch.cuda.synchronize()
+        # eager work between replays
+        junk = T.LaplacianFunction.apply(rhs[0], sbs[0].code).sum()
+        assert torch.isfinite(junk)
+        assert abs(le - lg) <= 1e-5 * max(1.0, abs(le)), (k, le, lg)
+        for p, q in zip(model.parameters(), model_g.parameters()):
+            assert torch.allclose(p, q, rtol=1e-5, atol=1e-6), k
+    # replaying again with the first right-hand sides gives the loss an eager step gives
+    le = T.train_step(model, sbs, steps[0], opt_e, act_dtype=act, loss_fn=T.relative_residual_sq_dot)
+    assert abs(le - float(graphed(steps[0]))) <= 1e-5 * max(1.0, abs(le))
+
+
+def test_graphed_steps_reduce_loss(dev):
+    T, model, sbs, gen, B = _setup(dev)
+    opt = torch.optim.Adam(model.parameters(), lr=3e-3, capturable=True)
+    rhs = [T.white_rhs(sb, 4, gen) for sb in sbs]
+    graphed = T.GraphedTrainStep(model, sbs, rhs, opt)
+    l0 = float(graphed(rhs))
+    for _ in range(30):
+        graphed(rhs)
+    l1 = float(graphed(rhs))
+    assert l1 < 0.7 * l0

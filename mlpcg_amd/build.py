@@ -42,8 +42,11 @@ def hipcc() -> str:
 
 
 # per-source flags: the fused apply keeps MFMA operands and results in VGPRs (no AGPR
-# copies in its inner loop)
-EXTRA = {"fused.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# copies in its inner loop), and its LDS reads stay 16-bit: without unaligned-access-mode
+# the compiler does not merge two neighbouring 16-bit reads at an odd element into one
+# misaligned 32-bit read, which measured 5x slower on gfx950
+EXTRA = {"fused.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form",
+                       "-Xclang", "-target-feature", "-Xclang", "-unaligned-access-mode"]}
 
 
 def _compile(src: str, verbose: bool) -> str:

@@ -48,9 +48,25 @@ template <> struct Mfma<f16> {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
   }
 };
-__host__ __device__ constexpr int nkc(int C) { return (27 * C + 31) / 32; }
+__host__ __device__ constexpr int nkc(int C) { return C; }
 
-// LDS offset of the neighbour d_q (q = 9(d1+1) + 3(d2+1) + (d3+1)) in the halo tile.
+// K slots.  Slot (g, j) of a 32-slot chunk (lane group g = lane / 16 holds slots 8g..8g+7
+// of the MFMA operands) carries tap fused_tap(g, j) (q = 9(d1+1) + 3(d2+1) + (d3+1)): the
+// (d1, d2) columns 2g and 2g+1 with d3 = -1, 0, 1 in j = 0..5, and two taps of column 8 in
+// j = 6, 7 (d3 = -1, 0 for g = 0; 0, 1 for g = 1).  Each lane thus reads three runs of
+// consecutive cells, one base register each, with the rest in instruction offsets.  The
+// five slots that repeat a tap or are spare (g = 1, j = 6; g = 2, 3, j = 6, 7) read valid
+// cells and get zero weights.  One chunk per image channel; the same table orders the
+// rows of D (the taps of K), so the x patches of the second product use the same bases.
+__host__ __device__ constexpr int fused_tap(int g, int j) {
+  return j < 6 ? 3 * (2 * g + j / 3) + j % 3 : 24 + (g == 1 ? 1 : 0) + (j - 6);
+}
+__host__ __device__ constexpr bool fused_slot_valid(int g, int j) { return j < 6 || g == 0 || (g == 1 && j == 7); }
+// D row rho is slot (g, j) of the second product's K: rho = 4g + j (j < 4), 16 + 4g + j - 4.
+__host__ __device__ constexpr int row_g(int rho) { return (rho & 15) >> 2; }
+__host__ __device__ constexpr int row_j(int rho) { return (rho & 3) + (rho >= 16 ? 4 : 0); }
+
+// LDS offset of the neighbour d_q in the halo tile.
 __device__ __forceinline__ int tap_off(int q) {
   return ((q / 9 - 1) * kPj + ((q / 3) % 3 - 1)) * kPz + (q % 3 - 1);
 }
@@ -61,7 +77,9 @@ size_t fused_weights_bytes(int C) {
   return (size_t)2 * nkc(C) * 64 * 8 * 2 + (size_t)2 * 64 * 4 * sizeof(float);
 }
 
-// apack[h][kc][lane][8] (FP16), then binit[h][lane][4] (float).
+// apack[h][kc][lane][8] (FP16): A[m][k] = W[tap of D row 16h + m][channel kc][tap of slot
+// k], zero for repeated / spare slots; then binit[h][lane][4] (float): the bias of D row
+// 16h + 4 (lane / 16) + i.
 __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __restrict__ W,
                                                               const float* __restrict__ b, int C,
                                                               f16* __restrict__ apack,
@@ -70,14 +88,15 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
   const int total = 2 * NK * 64 * 8;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int j = e % 8, lane = (e / 8) % 64, kc = (e / 512) % NK, h = e / (512 * NK);
-    const int m = 16 * h + lane % 16, k = 32 * kc + 8 * (lane / 16) + j;
-    const float w = (m < 27 && k < 27 * C) ? W[((long)m * C + k / 27) * 27 + k % 27] : 0.f;
-    apack[e] = (f16)w;
+    const int rho = 16 * h + lane % 16, gk = lane / 16;
+    const bool ok = fused_slot_valid(row_g(rho), row_j(rho)) && fused_slot_valid(gk, j);
+    const int p = fused_tap(row_g(rho), row_j(rho)), q = fused_tap(gk, j);
+    apack[e] = (f16)(ok ? W[((long)p * C + kc) * 27 + q] : 0.f);
   }
   for (int e = threadIdx.x; e < 2 * 64 * 4; e += blockDim.x) {
     const int i = e % 4, lane = (e / 4) % 64, h = e / 256;
-    const int m = 16 * h + 4 * (lane / 16) + i;
-    binit[e] = m < 27 ? b[m] : 0.f;
+    const int rho = 16 * h + 4 * (lane / 16) + i;
+    binit[e] = fused_slot_valid(row_g(rho), row_j(rho)) ? b[fused_tap(row_g(rho), row_j(rho))] : 0.f;
   }
 }
 
@@ -170,20 +189,9 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     }
   }
 
-  // per-lane LDS offsets: image inputs k of this lane's B rows; taps tau(g, j) of its D rows
-  int boff[NK][8], xoff[8];
-#pragma unroll
-  for (int kc = 0; kc < NK; ++kc)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = 32 * kc + 8 * g + j;
-      boff[kc][j] = k < 27 * C ? (k / 27) * kTile + tap_off(k % 27) : 0;  // padding reads any cell
-    }
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int m = j < 4 ? 4 * g + j : 16 + 4 * g + j - 4;
-    xoff[j] = m < 27 ? tap_off(m) : 0;  // rows m >= 27 of D are exactly zero
-  }
+  // per-lane LDS bases of the three runs of slots (j = 0-2, 3-5, 6-7) of lane group g; the
+  // same slots order the image inputs of the first product and the D rows of the second
+  const int bs[3] = {tap_off(fused_tap(g, 0)), tap_off(fused_tap(g, 3)), tap_off(fused_tap(g, 6))};
   // x rows of group t are read for voxel sigma_t(nn); this lane's result is group tstar
   int xsig[4];
 #pragma unroll
@@ -230,14 +238,14 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
       for (int kc = 0; kc < NK; ++kc) {
         h8 bv;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) bv[j] = is[ci + boff[kc][j]];
+        for (int j = 0; j < 8; ++j) bv[j] = is[kc * kTile + ci + bs[j < 3 ? 0 : (j < 6 ? 1 : 2)] + (j < 6 ? j % 3 : j - 6)];
         d0 = Mfma<f16>::run(A[0][kc], bv, d0);
         d1 = Mfma<f16>::run(A[1][kc], bv, d1);
       }
       const int cx = cb + xsig[tt];
       v8 ax, bk;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) ax[j] = xs[cx + xoff[j]];
+      for (int j = 0; j < 8; ++j) ax[j] = xs[cx + bs[j < 3 ? 0 : (j < 6 ? 1 : 2)] + (j < 6 ? j % 3 : j - 6)];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         bk[j] = (bf16)d0[j];

@@ -51,20 +51,54 @@ template <> struct Mfma<f16> {
 __host__ __device__ constexpr int nkc(int C) { return C; }
 
 // K slots.  Slot (g, j) of a 32-slot chunk (lane group g = lane / 16 holds slots 8g..8g+7
-// of the MFMA operands) carries tap fused_tap(g, j) (q = 9(d1+1) + 3(d2+1) + (d3+1)): the
-// (d1, d2) columns 2g and 2g+1 with d3 = -1, 0, 1 in j = 0..5, and two taps of column 8 in
-// j = 6, 7 (d3 = -1, 0 for g = 0; 0, 1 for g = 1).  Each lane thus reads three runs of
-// consecutive cells, one base register each, with the rest in instruction offsets.  The
-// five slots that repeat a tap or are spare (g = 1, j = 6; g = 2, 3, j = 6, 7) read valid
-// cells and get zero weights.  One chunk per image channel; the same table orders the
-// rows of D (the taps of K), so the x patches of the second product use the same bases.
+// of the MFMA operands) carries tap fused_tap(g, j) (q = 9(d1+1) + 3(d2+1) + (d3+1)): two
+// (d1, d2) columns c = 3(d1+1) + (d2+1) with d3 = -1, 0, 1 in j = 0..5, and two taps of
+// column 8 in j = 6, 7 (d3 = -1, 0 for g = 0; 0, 1 for g = 1).  Each lane thus reads three
+// runs of consecutive cells, one base register each, with the rest in instruction
+// offsets.  The five slots that repeat a tap or are spare (g = 1, j = 6; g = 2, 3, j = 6,
+// 7) read valid cells and get zero weights.  Columns go to lane groups as
+// (0, 1), (3, 5), (4, 2), (6, 7): for every read instruction the four groups' 16-cell
+// windows then fall on disjoint LDS banks (32 or 64 banks of 4 bytes), except in the
+// column-8 run where they read the same words.  One chunk per image channel; the same
+// table orders the rows of D (the taps of K), so the x patches of the second product use
+// the same bases.
+__host__ __device__ constexpr int fused_col(int i) { return i == 2 ? 3 : (i == 3 ? 5 : (i == 5 ? 2 : i)); }
 __host__ __device__ constexpr int fused_tap(int g, int j) {
-  return j < 6 ? 3 * (2 * g + j / 3) + j % 3 : 24 + (g == 1 ? 1 : 0) + (j - 6);
+  return j < 6 ? 3 * fused_col(2 * g + j / 3) + j % 3 : 24 + (g == 1 ? 1 : 0) + (j - 6);
 }
 __host__ __device__ constexpr bool fused_slot_valid(int g, int j) { return j < 6 || g == 0 || (g == 1 && j == 7); }
 // D row rho is slot (g, j) of the second product's K: rho = 4g + j (j < 4), 16 + 4g + j - 4.
 __host__ __device__ constexpr int row_g(int rho) { return (rho & 15) >> 2; }
 __host__ __device__ constexpr int row_j(int rho) { return (rho & 3) + (rho >= 16 ? 4 : 0); }
+
+// base[off] through a uniform base pointer and a 32-bit lane byte offset.  The empty asm
+// makes the offset opaque inside the loop body, so the compiler addresses with the scalar
+// base plus the 32-bit offset instead of a per-lane 64-bit address.
+template <typename T>
+__device__ __forceinline__ T lane_at(const T* base, unsigned boff) {
+  asm volatile("" : "+v"(boff));
+  return *(const T*)((const char*)base + boff);
+}
+template <typename T>
+__device__ __forceinline__ void lane_store(T* base, unsigned boff, T v) {
+  asm volatile("" : "+v"(boff));
+  *(T*)((char*)base + boff) = v;
+}
+
+// Buffer resource over `bytes` bytes from p (raw, stride 0): loads at byte offsets outside
+// [0, bytes) return 0 without touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ bf16 rsrc_load(__amdgpu_buffer_rsrc_t r, int boff, const float*) {
+  return (bf16)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+}
+__device__ __forceinline__ bf16 rsrc_load(__amdgpu_buffer_rsrc_t r, int boff, const bf16*) {
+  return __builtin_bit_cast(bf16, __builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0));
+}
+__device__ __forceinline__ f16 rsrc_load(__amdgpu_buffer_rsrc_t r, int boff, const f16*) {
+  return __builtin_bit_cast(f16, __builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0));
+}
 
 // LDS offset of the neighbour d_q in the halo tile.
 __device__ __forceinline__ int tap_off(int q) {
@@ -115,7 +149,9 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   __shared__ bf16 xs[kTile];
   __shared__ f16 is[C * kTile];
   if (solve_done(a.done)) return;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, nn = lane & 15;
+  // the wave index is uniform: keep it (and everything derived from it) in scalar registers
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = lane >> 4, nn = lane & 15;
   const int tz = a.n3 / kFz, tj = a.n2 / kFj;
   const unsigned t = xcd_block(blockIdx.x, gridDim.x);
   const int z0 = (int)(t % tz) * kFz;
@@ -137,13 +173,17 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   const float c0 = a.prev ? *(const float*)a.c0 : 0.f;
   const float c1 = a.prev ? *(const float*)a.c1 : 0.f;
 
-  // stage x and img of the halo tile (BF16); zero outside the grid.  Branch-free: every
-  // lane loads from a clamped in-grid address and selects zero, so all loads of a batch
-  // are in flight together before the LDS writes.  Rows r = w + 4 k (k < 25) of the
-  // 10 x 10 rows; element e = lane of each row, and e = 64 + lane for lanes 0 and 1.
+  // stage x and img of the halo tile (BF16); zero outside the grid.  Each row of the tile
+  // is read through a buffer resource over its in-grid part (none for rows outside the
+  // grid): the range check returns 0 for the halo cells, so there are no clamps or
+  // selects, and the lane offsets are the same for every row.  Rows r = w + 4 k (k < 25)
+  // of the 10 x 10 rows; element e = lane of each row, and e = 64 + lane for lanes 0, 1.
   const TX* __restrict__ x = (const TX*)a.x;
   const f16* __restrict__ img = (const f16*)a.img;
   static_assert(kPi * kPj == 100 && kPz == 66, "staging assumes 100 rows of 66 cells");
+  int zb[2];  // byte offsets of the lane's cells in a row (negative wraps: out of range)
+#pragma unroll
+  for (int u = 0; u < 2; ++u) zb[u] = (z0 - 1 + (u == 0 ? lane : 64 + (lane & 1))) * (int)sizeof(TX);
 #pragma unroll
   for (int k0 = 0; k0 < 25; k0 += 13) {
     constexpr int KB = 13;
@@ -157,18 +197,17 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
         const int gi = i0 + pi - 1, gj = j0 + pj - 1;
         const bool rok = gi >= 0 && gi < a.n1 && gj >= 0 && gj < a.n2;
         const long rb = ((long)min(max(gi, 0), a.n1 - 1) * a.n2 + min(max(gj, 0), a.n2 - 1)) * a.n3;
+        const int xbytes = rok ? a.n3 * (int)sizeof(TX) : 0;
+        const auto xr = row_rsrc(x + rb, xbytes);
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int e = u == 0 ? lane : 64 + (lane & 1);
-          const int gz = z0 - 1 + e;
-          const bool ok = rok && gz >= 0 && gz < a.n3;
-          const long gv = rb + min(max(gz, 0), a.n3 - 1);
-          const bf16 xr = (bf16)x[gv];
-          xv[kk][u] = ok ? xr : (bf16)0.f;
+        for (int u = 0; u < 2; ++u) xv[kk][u] = rsrc_load(xr, zb[u], (const TX*)nullptr);
 #pragma unroll
-          for (int c = 0; c < C; ++c) {
-            const f16 ir = img[c * V + gv];
-            iv[kk][c][u] = ok ? ir : (f16)0.f;
+        for (int c = 0; c < C; ++c) {
+          const auto ir = row_rsrc(img + c * V + rb, rok ? a.n3 * 2 : 0);
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int zh = sizeof(TX) == 2 ? zb[u] : zb[u] / 2;
+            iv[kk][c][u] = rsrc_load(ir, zh, (const f16*)nullptr);
           }
         }
       }
@@ -200,9 +239,10 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   const int myz = 16 * tstar + nn;
   __syncthreads();
 
-  TY* __restrict__ y = (TY*)a.y;
   // epilogue operands through always-valid pointers (x has at least 2V bytes), so the
-  // loads are unconditional and can be issued one chunk ahead
+  // loads are unconditional and can be issued one chunk ahead.  Row pointers are uniform;
+  // the lane's element offset myz is the same for every chunk.
+  TY* __restrict__ y = (TY*)a.y;
   const uint8_t* mA = a.mask_a ? a.mask_a : (const uint8_t*)a.x;
   const uint8_t* mB = a.mask_b ? a.mask_b : (const uint8_t*)a.x;
   const TP* pP = a.prev ? (const TP*)a.prev : (const TP*)a.x;
@@ -210,25 +250,28 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   const bool dots = a.partials != nullptr;
   const float* d0p = a.dot0 ? a.dot0 : (const float*)a.x;
   const float* d1p = a.dot1 ? a.dot1 : (const float*)a.x;
-  const long dmask = a.dot1 ? ~0L : 0L;  // without dot1 every lane reads element 0
+  const long rstep1 = a.dot1 ? 1 : 0;               // without dot1 every lane reads element 0
+  const unsigned mob = (unsigned)myz, mop = mob * (unsigned)sizeof(TP), mof = mob * 4u;
+  const unsigned moy = mob * (unsigned)sizeof(TY), d1off = a.dot1 ? mof : 0u;
   double red[2] = {0.0, 0.0};
   const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
-  auto vox = [&](int q) {
+  auto row = [&](int q) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
-    return ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0 + myz;
+    return ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0;
   };
-  long v = vox(0);
-  uint8_t ka = mA[v], kb = mB[v];
-  TP pv = pP[v];
-  float da = dots ? d0p[v] : 0.f, db = dots ? d1p[v & dmask] : 0.f;
+  long vr = row(0);
+  uint8_t ka = lane_at(mA + vr, mob), kb = lane_at(mB + vr, mob);
+  TP pv = lane_at(pP + vr, mop);
+  float da = dots ? lane_at(d0p + vr, mof) : 0.f, db = dots ? lane_at(d1p + vr * rstep1, d1off) : 0.f;
   for (int q = 0; q < kFi * 2; ++q) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
     const int cb = ((ii + 1) * kPj + (jj + 1)) * kPz + 1;
     // next chunk's epilogue operands
-    const long vn = vox(q + 1 < kFi * 2 ? q + 1 : q);
-    const uint8_t ka_n = mA[vn], kb_n = mB[vn];
-    const TP pv_n = pP[vn];
-    const float da_n = dots ? d0p[vn] : 0.f, db_n = dots ? d1p[vn & dmask] : 0.f;
+    const long vn = row(q + 1 < kFi * 2 ? q + 1 : q);
+    const uint8_t ka_n = lane_at(mA + vn, mob), kb_n = lane_at(mB + vn, mob);
+    const TP pv_n = lane_at(pP + vn, mop);
+    const float da_n = dots ? lane_at(d0p + vn, mof) : 0.f;
+    const float db_n = dots ? lane_at(d1p + vn * rstep1, d1off) : 0.f;
     f32x4 c2[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) {
@@ -264,12 +307,12 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     if (has_p) val = c0 * (float)pv + c1 * val;
     if (has_b && kb == 0) val = 0.f;
     const TY yv = (TY)val;
-    y[v] = yv;
+    lane_store(y + vr, moy, yv);
     if (dots) {
       red[0] += (double)(float)yv * (double)da;
       red[1] += (double)(float)yv * (double)db;
     }
-    v = vn;
+    vr = vn;
     ka = ka_n;
     kb = kb_n;
     pv = pv_n;
@@ -282,7 +325,7 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
 unsigned fused_blocks(int n1, int n2, int n3) { return (unsigned)((n1 / kFi) * (n2 / kFj) * (n3 / kFz)); }
 
 bool fused_eligible(int C, int B, int n1, int n2, int n3) {
-  return (C == 1 || C == 2) && B == 1 && n1 % kFi == 0 && n2 % kFj == 0 && n3 % kFz == 0;
+  return (C == 1 || C == 2) && B == 1 && n1 % kFi == 0 && n2 % kFj == 0 && n3 % kFz == 0 && n3 < (1 << 24);
 }
 
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
 // in lane (g, n) with g = (n/4 + t) mod 4, element n % 4: over the 4 groups of a 64-voxel
 // chunk every lane ends up holding exactly one voxel, z0 + 16 ((g - n/4) mod 4) + n.
 template <typename TX, typename TY, typename TP, int C>
-__global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void sm_fused_kernel(FusedArgs a) {
   typedef Mfma<bf16>::v8 v8;
   typedef Mfma<f16>::v8 h8;
   constexpr int NK = nkc(C);

@@ -437,6 +437,17 @@ static int pick_vz(int n3, int maxvz) {
   return 1;
 }
 
+// Small launches (the coarse levels) are latency-bound chains of memory round trips:
+// shorter runs per thread give more threads for the same voxels.  Halve the run length
+// while a launch would have fewer than kMinItems threads.
+constexpr long kMinItems = 1L << 18;
+
+static int pick_vz_small(int n3, int maxvz, long voxels) {
+  int vz = pick_vz(n3, maxvz);
+  while (vz > 1 && voxels / vz < kMinItems) vz >>= 1;
+  return vz;
+}
+
 // Launches over fewer voxels than this skip the solver's done flag: their run time is a
 // chain of memory round trips, and the flag load would add one; the work a speculative
 // iteration then does on them is a few KB.
@@ -520,7 +531,7 @@ int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& args, hipStream_t s
   a.done = done_for(a.done, (long)a.n1 * a.n2 * a.n3);
   // 16-byte stencil loads per tap and thread: VZ = 8 (16-bit), 4 (FP32), 2 (FP64); wider
   // runs hoist more loads than the register file holds at two waves per SIMD
-  const int vz = pick_vz(a.n3, tk == kF64 ? 2 : (tk == kF32 ? 4 : 8));
+  const int vz = pick_vz_small(a.n3, tk == kF64 ? 2 : (tk == kF32 ? 4 : 8), (long)a.n1 * a.n2 * a.n3 * a.B);
   if (tk == kF64) {
     MLPCG_REQUIRE(tx == kF64 && ty == kF64, "sm_apply: f64 stencil needs f64 vectors");
     launch_apply_t<double, double, double, double>(a, vz, st);
@@ -548,7 +559,7 @@ int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& args, hipStream_t s
 template <typename T>
 static void launch_pool_t(const void* in, void* out, int B, int n1, int n2, int n3, const int* done,
                           hipStream_t st) {
-  const int vz = pick_vz(n3, 4);
+  const int vz = pick_vz_small(n3, 4, (long)n1 * n2 * n3 * B);
   const long items = (long)n1 * n2 * (n3 / vz);
   if (vz == 4)
     avgpool2_kernel<T, T, 4><<<grid1(items, B), kThreads, 0, st>>>((const T*)in, (T*)out, B, n1, n2, n3, done);
@@ -576,7 +587,7 @@ int launch_avgpool2(int dtype, const void* in, void* out, int B, int n1, int n2,
 
 template <typename TI, typename TO>
 static void launch_up_t(const UpArgs& a, hipStream_t st) {
-  const int vz = pick_vz(a.n3, 8);
+  const int vz = pick_vz_small(a.n3, 8, (long)a.n1 * a.n2 * a.n3 * a.B);
   const long items = (long)a.n1 * a.n2 * (a.n3 / vz);
   if (vz == 8) upsample2_kernel<TI, TO, TO, 8><<<grid1(items, a.B), kThreads, 0, st>>>(a);
   else if (vz == 4) upsample2_kernel<TI, TO, TO, 4><<<grid1(items, a.B), kThreads, 0, st>>>(a);

@@ -50,26 +50,42 @@ template <> struct Mfma<f16> {
 };
 __host__ __device__ constexpr int nkc(int C) { return C; }
 
-// K slots.  Slot (g, j) of a 32-slot chunk (lane group g = lane / 16 holds slots 8g..8g+7
-// of the MFMA operands) carries tap fused_tap(g, j) (q = 9(d1+1) + 3(d2+1) + (d3+1)): two
-// (d1, d2) columns c = 3(d1+1) + (d2+1) with d3 = -1, 0, 1 in j = 0..5, and two taps of
-// column 8 in j = 6, 7 (d3 = -1, 0 for g = 0; 0, 1 for g = 1).  Each lane thus reads three
-// runs of consecutive cells, one base register each, with the rest in instruction
-// offsets.  The five slots that repeat a tap or are spare (g = 1, j = 6; g = 2, 3, j = 6,
-// 7) read valid cells and get zero weights.  Columns go to lane groups as
-// (0, 1), (3, 5), (4, 2), (6, 7): for every read instruction the four groups' 16-cell
-// windows then fall on disjoint LDS banks (32 or 64 banks of 4 bytes), except in the
-// column-8 run where they read the same words.  One chunk per image channel; the same
-// table orders the rows of D (the taps of K), so the x patches of the second product use
-// the same bases.
+// K slots.  A 64-voxel chunk of a z-row is processed as four groups of 16 voxels that
+// share a z parity: group t holds z = 32 (t / 2) + 2n + (t % 2), n = 0..15.  With one
+// parity per group, pairs of neighbouring taps along z sit at 4-byte aligned LDS
+// addresses for every lane, so a lane's eight K slots come from four 32-bit reads.
+// Slot (g, j) of a 32-slot chunk (lane group g = lane / 16 holds slots 8g..8g+7 of the
+// MFMA operands) carries tap fused_tap(par, g, j) (q = 9(d1+1) + 3(d2+1) + (d3+1) =
+// 3c + d3 + 1 for the (d1, d2) column c):
+//   even z: j = 0..5: (c1,-1) (c1,0) (c1,+1) (c2,+1) (c2,-1) (c2,0); j = 6, 7: (8,-1) (8,0)
+//           for g = 0, (8,+1) (spare) for g = 1;
+//   odd z:  j = 0..5: (c1,0) (c1,+1) (c1,-1) (c2,-1) (c2,0) (c2,+1); j = 6, 7: (8,0) (8,+1)
+//           for g = 0, (spare) (8,-1) for g = 1;
+// lane groups 2 and 3 have spares in j = 6, 7.  Registers hold slot pairs (2i, 2i+1): the
+// pairs (c1,-1)(c1,0), (c2,-1)(c2,0) of an even voxel and (c1,0)(c1,+1), (c2,0)(c2,+1) of
+// an odd voxel are aligned words, and the remaining pair is put together from the halves
+// of two words that the other parity reads as its pairs.  Spare slots read valid cells
+// and get zero weights.  Columns go to lane groups as (c1, c2) = (0, 1), (3, 5), (4, 2),
+// (6, 7).  One chunk per image channel; the same table orders the rows of D (the taps of
+// K), so the x patches of the second product use the same reads.
 __host__ __device__ constexpr int fused_col(int i) { return i == 2 ? 3 : (i == 3 ? 5 : (i == 5 ? 2 : i)); }
-__host__ __device__ constexpr int fused_tap(int g, int j) {
-  return j < 6 ? 3 * fused_col(2 * g + j / 3) + j % 3 : 24 + (g == 1 ? 1 : 0) + (j - 6);
+__host__ __device__ constexpr int fused_tap(int par, int g, int j) {
+  const int c1 = fused_col(2 * g), c2 = fused_col(2 * g + 1);
+  if (par == 0) {
+    return j == 0 ? 3 * c1 : j == 1 ? 3 * c1 + 1 : j == 2 ? 3 * c1 + 2 : j == 3 ? 3 * c2 + 2 :
+           j == 4 ? 3 * c2 : j == 5 ? 3 * c2 + 1 : j == 6 ? (g == 1 ? 26 : 24) : 25;
+  }
+  return j == 0 ? 3 * c1 + 1 : j == 1 ? 3 * c1 + 2 : j == 2 ? 3 * c1 : j == 3 ? 3 * c2 :
+         j == 4 ? 3 * c2 + 1 : j == 5 ? 3 * c2 + 2 : j == 6 ? 25 : (g == 1 ? 24 : 26);
 }
-__host__ __device__ constexpr bool fused_slot_valid(int g, int j) { return j < 6 || g == 0 || (g == 1 && j == 7); }
+__host__ __device__ constexpr bool fused_slot_valid(int par, int g, int j) {
+  return j < 6 || g == 0 || (g == 1 && j == (par == 0 ? 6 : 7));
+}
 // D row rho is slot (g, j) of the second product's K: rho = 4g + j (j < 4), 16 + 4g + j - 4.
 __host__ __device__ constexpr int row_g(int rho) { return (rho & 15) >> 2; }
 __host__ __device__ constexpr int row_j(int rho) { return (rho & 3) + (rho >= 16 ? 4 : 0); }
+// LDS element offset of the (d1, d2) column c in the halo tile
+__host__ __device__ constexpr int col_off(int c) { return (c / 3 - 1) * kPj * kPz + (c % 3 - 1) * kPz; }
 
 // base[off] through a uniform base pointer and a 32-bit lane byte offset.  The empty asm
 // makes the offset opaque inside the loop body, so the compiler addresses with the scalar
@@ -100,54 +116,61 @@ __device__ __forceinline__ f16 rsrc_load(__amdgpu_buffer_rsrc_t r, int boff, con
   return __builtin_bit_cast(f16, __builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0));
 }
 
-// LDS offset of the neighbour d_q in the halo tile.
-__device__ __forceinline__ int tap_off(int q) {
-  return ((q / 9 - 1) * kPj + ((q / 3) % 3 - 1)) * kPz + (q % 3 - 1);
+// 32-bit LDS word at an even element index (both 16-bit halves)
+template <typename T>
+__device__ __forceinline__ uint32_t lds_word(const T* base, int elem) {
+  return *(const uint32_t*)(base + elem);
 }
+// low halves of a and b (a in the low half of the result); high halves likewise
+__device__ __forceinline__ uint32_t pack_lo(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
+__device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x07060302u); }
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 }  // namespace
 
-size_t fused_weights_bytes(int C) {
-  return (size_t)2 * nkc(C) * 64 * 8 * 2 + (size_t)2 * 64 * 4 * sizeof(float);
-}
+size_t fused_bias_offset(int C) { return (size_t)2 * 2 * nkc(C) * 64 * 8 * sizeof(f16); }
 
-// apack[h][kc][lane][8] (FP16): A[m][k] = W[tap of D row 16h + m][channel kc][tap of slot
-// k], zero for repeated / spare slots; then binit[h][lane][4] (float): the bias of D row
-// 16h + 4 (lane / 16) + i.
+size_t fused_weights_bytes(int C) { return fused_bias_offset(C) + (size_t)2 * 2 * 64 * 4 * sizeof(float); }
+
+// apack[par][h][kc][lane][8] (FP16): A[m][k] = W[tap of D row 16h + m][channel kc][tap of
+// slot k] for z parity par, zero for spare slots; then binit[par][h][lane][4] (float): the
+// bias of D row 16h + 4 (lane / 16) + i.
 __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __restrict__ W,
                                                               const float* __restrict__ b, int C,
                                                               f16* __restrict__ apack,
                                                               float* __restrict__ binit) {
   const int NK = nkc(C);
-  const int total = 2 * NK * 64 * 8;
+  const int total = 2 * 2 * NK * 64 * 8;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int j = e % 8, lane = (e / 8) % 64, kc = (e / 512) % NK, h = e / (512 * NK);
+    const int j = e % 8, lane = (e / 8) % 64, kc = (e / 512) % NK, h = (e / (512 * NK)) % 2;
+    const int par = e / (1024 * NK);
     const int rho = 16 * h + lane % 16, gk = lane / 16;
-    const bool ok = fused_slot_valid(row_g(rho), row_j(rho)) && fused_slot_valid(gk, j);
-    const int p = fused_tap(row_g(rho), row_j(rho)), q = fused_tap(gk, j);
+    const bool ok = fused_slot_valid(par, row_g(rho), row_j(rho)) && fused_slot_valid(par, gk, j);
+    const int p = fused_tap(par, row_g(rho), row_j(rho)), q = fused_tap(par, gk, j);
     apack[e] = (f16)(ok ? W[((long)p * C + kc) * 27 + q] : 0.f);
   }
-  for (int e = threadIdx.x; e < 2 * 64 * 4; e += blockDim.x) {
-    const int i = e % 4, lane = (e / 4) % 64, h = e / 256;
+  for (int e = threadIdx.x; e < 2 * 2 * 64 * 4; e += blockDim.x) {
+    const int i = e % 4, lane = (e / 4) % 64, h = (e / 256) % 2, par = e / 512;
     const int rho = 16 * h + 4 * (lane / 16) + i;
-    binit[e] = fused_slot_valid(row_g(rho), row_j(rho)) ? b[fused_tap(row_g(rho), row_j(rho))] : 0.f;
+    binit[e] = fused_slot_valid(par, row_g(rho), row_j(rho)) ? b[fused_tap(par, row_g(rho), row_j(rho))] : 0.f;
   }
 }
 
 // The apply is a second MFMA: after the first, lane (g, n) (g = l / 16, n = l % 16) holds
-// D = K for the taps tau(g, j) = 4g + j (j < 4) and 16 + 4g + j - 4 (j >= 4) of voxel n.
-// With A2[m][k] = x(voxel sigma(m), tap tau(k)) and B2[k][n] = bf16(D[tau(k)][n]),
-// C2[m][n] = sum_k x(sigma(m), tau k) K_{tau k}(n) is y(n) where sigma(m) = n.  The row
-// permutation sigma_t(m) = 4 ((m/4 - t) mod 4) + m % 4 of group t places y(n) of group t
-// in lane (g, n) with g = (n/4 + t) mod 4, element n % 4: over the 4 groups of a 64-voxel
-// chunk every lane ends up holding exactly one voxel, z0 + 16 ((g - n/4) mod 4) + n.
+// D = K for the taps of rows tau(g, j) = 4g + j (j < 4) and 16 + 4g + j - 4 (j >= 4) of
+// voxel n of the group.  With A2[m][k] = x(voxel sigma(m), tap of row tau(k)) and
+// B2[k][n] = bf16(D[tau(k)][n]), C2[m][n] = sum_k x(sigma(m), .) K(n) is y(n) where
+// sigma(m) = n.  The row permutation sigma_t(m) = 4 ((m/4 - t) mod 4) + m % 4 of group t
+// places y(n) of group t in lane (g, n) with g = (n/4 + t) mod 4, element n % 4: over the
+// 4 groups of a 64-voxel chunk every lane ends up holding exactly one voxel, voxel n of
+// group t* = (g - n/4) mod 4, i.e. z0 + 32 (t*/2) + 2n + t* % 2.
 template <typename TX, typename TY, typename TP, int C>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void sm_fused_kernel(FusedArgs a) {
+__global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   typedef Mfma<bf16>::v8 v8;
   typedef Mfma<f16>::v8 h8;
   constexpr int NK = nkc(C);
-  __shared__ bf16 xs[kTile];
-  __shared__ f16 is[C * kTile];
+  __shared__ __attribute__((aligned(16))) bf16 xs[kTile];
+  __shared__ __attribute__((aligned(16))) f16 is[C * kTile];
   if (solve_done(a.done)) return;
   // the wave index is uniform: keep it (and everything derived from it) in scalar registers
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -161,14 +184,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
 
   // per-lane constant operands (global loads issued before the tile staging)
   const f16* apack = (const f16*)a.apack;
-  h8 A[2][NK];
+  h8 A[2][2][NK];  // [z parity][row half][channel]
+  f32x4 Bi[2][2];   // bias of the D rows [z parity][row half]
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+  for (int par = 0; par < 2; ++par)
 #pragma unroll
-    for (int kc = 0; kc < NK; ++kc) A[h][kc] = *(const h8*)(apack + (((long)h * NK + kc) * 64 + lane) * 8);
-  const float* binit = a.binit;
-  const f32x4 B0 = *(const f32x4*)(binit + lane * 4);
-  const f32x4 B1 = *(const f32x4*)(binit + (64 + lane) * 4);
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int kc = 0; kc < NK; ++kc)
+        A[par][h][kc] = *(const h8*)(apack + ((((long)par * 2 + h) * NK + kc) * 64 + lane) * 8);
+      Bi[par][h] = *(const f32x4*)(a.binit + ((par * 2 + h) * 64 + lane) * 4);
+    }
   const float scale = a.in_scale ? (float)(*a.in_scale) : 1.f;
   const float c0 = a.prev ? *(const float*)a.c0 : 0.f;
   const float c1 = a.prev ? *(const float*)a.c1 : 0.f;
@@ -228,15 +254,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     }
   }
 
-  // per-lane LDS bases of the three runs of slots (j = 0-2, 3-5, 6-7) of lane group g; the
-  // same slots order the image inputs of the first product and the D rows of the second
-  const int bs[3] = {tap_off(fused_tap(g, 0)), tap_off(fused_tap(g, 3)), tap_off(fused_tap(g, 6))};
-  // x rows of group t are read for voxel sigma_t(nn); this lane's result is group tstar
-  int xsig[4];
+  // per-lane LDS word offsets (even element indices relative to the chunk's cell cb + 32h):
+  // image words of this lane's voxel 2nn (even group; the odd group reads the same words),
+  // x words of voxel sigma_t(nn) of group t
+  const int o1 = col_off(fused_col(2 * g)) - 1, o2 = col_off(fused_col(2 * g + 1)) - 1, o8 = col_off(8) - 1;
+  const int r3e = g == 1 ? 2 : 0, r3o = g == 1 ? 0 : 2;  // column-8 word: even / odd groups
+  const int iP = o1 + 2 * nn, iQ = o2 + 2 * nn, i8e = o8 + 2 * nn + r3e, i8o = o8 + 2 * nn + r3o;
+  int xP[4], xQ[4], x8[4];
 #pragma unroll
-  for (int tt = 0; tt < 4; ++tt) xsig[tt] = 16 * tt + 4 * (((nn >> 2) - tt) & 3) + (nn & 3);
+  for (int tt = 0; tt < 4; ++tt) {
+    const int sg = 4 * (((nn >> 2) - tt) & 3) + (nn & 3);
+    xP[tt] = o1 + 2 * sg;
+    xQ[tt] = o2 + 2 * sg;
+    x8[tt] = o8 + 2 * sg + ((tt & 1) ? r3o : r3e);
+  }
   const int tstar = (g - (nn >> 2)) & 3;
-  const int myz = 16 * tstar + nn;
+  const int myz = 32 * (tstar >> 1) + 2 * nn + (tstar & 1);
   __syncthreads();
 
   // epilogue operands through always-valid pointers (x has at least 2V bytes), so the
@@ -274,27 +307,41 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     const float db_n = dots ? lane_at(d1p + vn * rstep1, d1off) : 0.f;
     f32x4 c2[4];
 #pragma unroll
-    for (int tt = 0; tt < 4; ++tt) {
-      const int ci = cb + 16 * tt + nn;
-      f32x4 d0 = B0, d1 = B1;
+    for (int h = 0; h < 2; ++h) {
+      // image words shared by the even (t = 2h) and odd (t = 2h + 1) groups of this half
+      h8 bv[2][NK];
 #pragma unroll
       for (int kc = 0; kc < NK; ++kc) {
-        h8 bv;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) bv[j] = is[kc * kTile + ci + bs[j < 3 ? 0 : (j < 6 ? 1 : 2)] + (j < 6 ? j % 3 : j - 6)];
-        d0 = Mfma<f16>::run(A[0][kc], bv, d0);
-        d1 = Mfma<f16>::run(A[1][kc], bv, d1);
+        const f16* ib = is + kc * kTile + cb + 32 * h;
+        const uint32_t p0 = lds_word(ib, iP), p2 = lds_word(ib, iP + 2);
+        const uint32_t q0 = lds_word(ib, iQ), q2 = lds_word(ib, iQ + 2);
+        const uint32_t e8 = lds_word(ib, i8e), d8 = lds_word(ib, i8o);
+        bv[0][kc] = __builtin_bit_cast(h8, u32x4{p0, pack_lo(p2, q2), q0, e8});
+        bv[1][kc] = __builtin_bit_cast(h8, u32x4{p2, pack_hi(p0, q0), q2, d8});
       }
-      const int cx = cb + xsig[tt];
-      v8 ax, bk;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) ax[j] = xs[cx + bs[j < 3 ? 0 : (j < 6 ? 1 : 2)] + (j < 6 ? j % 3 : j - 6)];
+      for (int par = 0; par < 2; ++par) {
+        const int tt = 2 * h + par;
+        f32x4 d0 = Bi[par][0], d1 = Bi[par][1];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        bk[j] = (bf16)d0[j];
-        bk[j + 4] = (bf16)d1[j];
+        for (int kc = 0; kc < NK; ++kc) {
+          d0 = Mfma<f16>::run(A[par][0][kc], bv[par][kc], d0);
+          d1 = Mfma<f16>::run(A[par][1][kc], bv[par][kc], d1);
+        }
+        const bf16* xb = xs + cb + 32 * h;
+        const uint32_t p0 = lds_word(xb, xP[tt]), p2 = lds_word(xb, xP[tt] + 2);
+        const uint32_t q0 = lds_word(xb, xQ[tt]), q2 = lds_word(xb, xQ[tt] + 2);
+        const uint32_t w8 = lds_word(xb, x8[tt]);
+        const v8 ax = par == 0 ? __builtin_bit_cast(v8, u32x4{p0, pack_lo(p2, q2), q0, w8})
+                               : __builtin_bit_cast(v8, u32x4{p2, pack_hi(p0, q0), q2, w8});
+        v8 bk;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          bk[j] = (bf16)d0[j];
+          bk[j + 4] = (bf16)d1[j];
+        }
+        c2[tt] = Mfma<bf16>::run(ax, bk, zero4);
       }
-      c2[tt] = Mfma<bf16>::run(ax, bk, zero4);
     }
     // this lane's voxel: group tstar, element nn % 4 (selects, no branches)
     f32x4 cs = c2[0];
@@ -331,7 +378,7 @@ bool fused_eligible(int C, int B, int n1, int n2, int n3) {
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
   MLPCG_REQUIRE(C == 1 || C == 2, "fused_pack: C must be 1 or 2");
   MLPCG_REQUIRE(tw == kBF16, "fused_pack: BF16 plans only (got %d)", tw);
-  float* binit = (float*)((char*)wbuf + (size_t)2 * nkc(C) * 64 * 8 * 2);
+  float* binit = (float*)((char*)wbuf + fused_bias_offset(C));
   fused_pack_kernel<<<1, kThreads, 0, st>>>((const float*)W, (const float*)b, C, (f16*)wbuf, binit);
   return check_launch("fused_pack");
 }

@@ -62,6 +62,7 @@ struct FusedArgs {
 };
 
 size_t fused_weights_bytes(int C);
+size_t fused_bias_offset(int C);  // byte offset of the bias block in the packed weights
 unsigned fused_blocks(int n1, int n2, int n3);
 bool fused_eligible(int C, int B, int n1, int n2, int n3);
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st);

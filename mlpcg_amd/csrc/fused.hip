@@ -78,6 +78,8 @@ __host__ __device__ constexpr int fused_tap(int par, int g, int j) {
   return j == 0 ? 3 * c1 + 1 : j == 1 ? 3 * c1 + 2 : j == 2 ? 3 * c1 : j == 3 ? 3 * c2 :
          j == 4 ? 3 * c2 + 1 : j == 5 ? 3 * c2 + 2 : j == 6 ? 25 : (g == 1 ? 24 : 26);
 }
+constexpr int kBiasGroup = 2;             // lane group whose spare slots carry the bias
+constexpr uint32_t kOnesF16 = 0x3c003c00u;  // two FP16 ones
 __host__ __device__ constexpr bool fused_slot_valid(int par, int g, int j) {
   return j < 6 || g == 0 || (g == 1 && j == (par == 0 ? 6 : 7));
 }
@@ -128,31 +130,32 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 }  // namespace
 
-size_t fused_bias_offset(int C) { return (size_t)2 * 2 * nkc(C) * 64 * 8 * sizeof(f16); }
-
-size_t fused_weights_bytes(int C) { return fused_bias_offset(C) + (size_t)2 * 2 * 64 * 4 * sizeof(float); }
+size_t fused_weights_bytes(int C) { return (size_t)2 * 2 * nkc(C) * 64 * 8 * sizeof(f16); }
 
 // apack[par][h][kc][lane][8] (FP16): A[m][k] = W[tap of D row 16h + m][channel kc][tap of
-// slot k] for z parity par, zero for spare slots; then binit[par][h][lane][4] (float): the
-// bias of D row 16h + 4 (lane / 16) + i.
+// slot k] for z parity par, zero for spare slots.  The bias rides on the spare slots
+// (g, j) = (2, 6), (2, 7) of channel 0, whose image values the kernel sets to 1: their
+// weights are the bias split into an FP16 value and its FP16 remainder, so D gets the
+// bias to about 22 bits with FP32 accumulation.
 __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __restrict__ W,
                                                               const float* __restrict__ b, int C,
-                                                              f16* __restrict__ apack,
-                                                              float* __restrict__ binit) {
+                                                              f16* __restrict__ apack) {
   const int NK = nkc(C);
   const int total = 2 * 2 * NK * 64 * 8;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int j = e % 8, lane = (e / 8) % 64, kc = (e / 512) % NK, h = (e / (512 * NK)) % 2;
     const int par = e / (1024 * NK);
     const int rho = 16 * h + lane % 16, gk = lane / 16;
-    const bool ok = fused_slot_valid(par, row_g(rho), row_j(rho)) && fused_slot_valid(par, gk, j);
+    const bool rok = fused_slot_valid(par, row_g(rho), row_j(rho));
     const int p = fused_tap(par, row_g(rho), row_j(rho)), q = fused_tap(par, gk, j);
-    apack[e] = (f16)(ok ? W[((long)p * C + kc) * 27 + q] : 0.f);
-  }
-  for (int e = threadIdx.x; e < 2 * 2 * 64 * 4; e += blockDim.x) {
-    const int i = e % 4, lane = (e / 4) % 64, h = (e / 256) % 2, par = e / 512;
-    const int rho = 16 * h + 4 * (lane / 16) + i;
-    binit[e] = fused_slot_valid(par, row_g(rho), row_j(rho)) ? b[fused_tap(par, row_g(rho), row_j(rho))] : 0.f;
+    float v = 0.f;
+    if (rok && fused_slot_valid(par, gk, j)) {
+      v = W[((long)p * C + kc) * 27 + q];
+    } else if (rok && kc == 0 && gk == kBiasGroup) {
+      const float hi = (float)(f16)b[p];
+      v = j == 6 ? hi : (j == 7 ? b[p] - hi : 0.f);
+    }
+    apack[e] = (f16)v;
   }
 }
 
@@ -185,16 +188,14 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   // per-lane constant operands (global loads issued before the tile staging)
   const f16* apack = (const f16*)a.apack;
   h8 A[2][2][NK];  // [z parity][row half][channel]
-  f32x4 Bi[2][2];   // bias of the D rows [z parity][row half]
 #pragma unroll
   for (int par = 0; par < 2; ++par)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int kc = 0; kc < NK; ++kc)
         A[par][h][kc] = *(const h8*)(apack + ((((long)par * 2 + h) * NK + kc) * 64 + lane) * 8);
-      Bi[par][h] = *(const f32x4*)(a.binit + ((par * 2 + h) * 64 + lane) * 4);
-    }
+  const bool bias_lane = g == kBiasGroup;  // its column-8 word of channel 0 carries (1, 1)
   const float scale = a.in_scale ? (float)(*a.in_scale) : 1.f;
   const float c0 = a.prev ? *(const float*)a.c0 : 0.f;
   const float c1 = a.prev ? *(const float*)a.c1 : 0.f;
@@ -315,14 +316,18 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
         const f16* ib = is + kc * kTile + cb + 32 * h;
         const uint32_t p0 = lds_word(ib, iP), p2 = lds_word(ib, iP + 2);
         const uint32_t q0 = lds_word(ib, iQ), q2 = lds_word(ib, iQ + 2);
-        const uint32_t e8 = lds_word(ib, i8e), d8 = lds_word(ib, i8o);
+        uint32_t e8 = lds_word(ib, i8e), d8 = lds_word(ib, i8o);
+        if (kc == 0) {
+          e8 = bias_lane ? kOnesF16 : e8;
+          d8 = bias_lane ? kOnesF16 : d8;
+        }
         bv[0][kc] = __builtin_bit_cast(h8, u32x4{p0, pack_lo(p2, q2), q0, e8});
         bv[1][kc] = __builtin_bit_cast(h8, u32x4{p2, pack_hi(p0, q0), q2, d8});
       }
 #pragma unroll
       for (int par = 0; par < 2; ++par) {
         const int tt = 2 * h + par;
-        f32x4 d0 = Bi[par][0], d1 = Bi[par][1];
+        f32x4 d0 = zero4, d1 = zero4;
 #pragma unroll
         for (int kc = 0; kc < NK; ++kc) {
           d0 = Mfma<f16>::run(A[par][0][kc], bv[par][kc], d0);
@@ -378,8 +383,7 @@ bool fused_eligible(int C, int B, int n1, int n2, int n3) {
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
   MLPCG_REQUIRE(C == 1 || C == 2, "fused_pack: C must be 1 or 2");
   MLPCG_REQUIRE(tw == kBF16, "fused_pack: BF16 plans only (got %d)", tw);
-  float* binit = (float*)((char*)wbuf + fused_bias_offset(C));
-  fused_pack_kernel<<<1, kThreads, 0, st>>>((const float*)W, (const float*)b, C, (f16*)wbuf, binit);
+  fused_pack_kernel<<<1, kThreads, 0, st>>>((const float*)W, (const float*)b, C, (f16*)wbuf);
   return check_launch("fused_pack");
 }
 

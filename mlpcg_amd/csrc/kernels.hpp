@@ -42,8 +42,7 @@ struct UpArgs {
 // epilogue as ApplyArgs.  Grids with n1, n2 % 8 == 0 and n3 % 64 == 0, B = 1, C <= 2.
 struct FusedArgs {
   const void* img;
-  const void* apack;
-  const float* binit;
+  const void* apack;  // packed weights and bias (launch_fused_pack)
   const void* x;
   void* y;
   const void* prev;
@@ -62,7 +61,6 @@ struct FusedArgs {
 };
 
 size_t fused_weights_bytes(int C);
-size_t fused_bias_offset(int C);  // byte offset of the bias block in the packed weights
 unsigned fused_blocks(int n1, int n2, int n3);
 bool fused_eligible(int C, int B, int n1, int n2, int n3);
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st);

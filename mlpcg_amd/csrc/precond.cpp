@@ -205,7 +205,6 @@ int Precond::run_block(int k, int level, const void* x, int tx, void* y, int ty,
     f.partials = partials;
     f.img = imgw[level];
     f.apack = wpack[k];
-    f.binit = (const float*)((const char*)wpack[k] + fused_bias_offset(C));
     f.x = x;
     f.y = y;
     f.prev = prev;

@@ -270,6 +270,7 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     x8[tt] = o8 + 2 * sg + ((tt & 1) ? r3o : r3e);
   }
   const int tstar = (g - (nn >> 2)) & 3;
+  const bool odd_e = (nn & 1) != 0, high_e = (nn & 2) != 0;  // element nn % 4 of the result
   const int myz = 32 * (tstar >> 1) + 2 * nn + (tstar & 1);
   __syncthreads();
 
@@ -352,8 +353,8 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     f32x4 cs = c2[0];
 #pragma unroll
     for (int tt = 1; tt < 4; ++tt) cs = (tstar == tt) ? c2[tt] : cs;
-    const int e4 = nn & 3;
-    const float ymine = e4 == 0 ? cs[0] : (e4 == 1 ? cs[1] : (e4 == 2 ? cs[2] : cs[3]));
+    const float lo = odd_e ? cs[1] : cs[0], hi = odd_e ? cs[3] : cs[2];
+    const float ymine = high_e ? hi : lo;
     float val = ymine * scale;
     if (has_a && ka == 0) val = 0.f;
     if (has_p) val = c0 * (float)pv + c1 * val;

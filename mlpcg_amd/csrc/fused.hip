@@ -14,14 +14,15 @@
 // Operand layout of v_mfma_f32_16x16x32_{f16,bf16} (lane l, element j / i):
 //   A[l % 16][8 (l / 16) + j],  B[8 (l / 16) + j][l % 16],  D[4 (l / 16) + i][l % 16].
 // Precision: step 1 takes img and W in FP16 (11-bit significands: image values lie in
-// [0, 1] and weights are O(1), so the range is ample) with FP32 accumulation and the bias
-// as the FP32 initial accumulator, which makes K more accurate than a FP32-computed K
-// rounded to BF16; step 2 rounds x and K to BF16 (the storage type of the plan) and
-// accumulates in FP32.
+// [0, 1] and weights are O(1), so the range is ample) with FP32 accumulation, the bias
+// entering as an FP16 value plus its FP16 remainder on two spare K slots (about 22
+// significant bits), which makes K more accurate than a FP32-computed K rounded to BF16;
+// step 2 rounds x and K to BF16 (the storage type of the plan) and accumulates in FP32.
 //
-// Tile: a block of 4 waves covers 8 planes x 8 rows x 64 z-voxels; x and img of the tile
-// plus a one-cell halo are staged in LDS; a wave processes 64 consecutive z-voxels of one
-// (plane, row) per step: 4 groups of 16 voxels.
+// Tile: a block of 4 waves covers FI planes x 8 rows x 64 z-voxels (FI = 8, or 4 where
+// 8-plane tiles give too few blocks); x and img of the tile plus a one-cell halo are
+// staged in LDS; a wave processes 64 z-voxels of one (plane, row) per step: 4 groups of
+// 16 voxels.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -29,9 +30,9 @@ namespace mlpcg {
 
 namespace {
 
-constexpr int kFz = 64, kFj = 8, kFi = 8;
-constexpr int kPz = kFz + 2, kPj = kFj + 2, kPi = kFi + 2;
-constexpr int kTile = kPi * kPj * kPz;  // halo-tile cells
+constexpr int kFz = 64, kFj = 8;  // tile: FI planes (8, or 4 on small grids) x 8 rows x 64 z
+constexpr int kPz = kFz + 2, kPj = kFj + 2;
+__host__ __device__ constexpr int tile_cells(int FI) { return (FI + 2) * kPj * kPz; }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -167,8 +168,10 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
 // places y(n) of group t in lane (g, n) with g = (n/4 + t) mod 4, element n % 4: over the
 // 4 groups of a 64-voxel chunk every lane ends up holding exactly one voxel, voxel n of
 // group t* = (g - n/4) mod 4, i.e. z0 + 32 (t*/2) + 2n + t* % 2.
-template <typename TX, typename TY, typename TP, int C>
+template <typename TX, typename TY, typename TP, int C, int FI>
 __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
+  constexpr int kTile = tile_cells(FI);
+  constexpr int kRows = (FI + 2) * kPj / 4;  // halo-tile rows staged per wave
   typedef Mfma<bf16>::v8 v8;
   typedef Mfma<f16>::v8 h8;
   constexpr int NK = nkc(C);
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   const unsigned t = xcd_block(blockIdx.x, gridDim.x);
   const int z0 = (int)(t % tz) * kFz;
   const int j0 = (int)((t / tz) % tj) * kFj;
-  const int i0 = (int)(t / tz / tj) * kFi;
+  const int i0 = (int)(t / tz / tj) * FI;
   const long V = (long)a.n1 * a.n2 * a.n3;
 
   // per-lane constant operands (global loads issued before the tile staging)
@@ -203,23 +206,24 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   // stage x and img of the halo tile (BF16); zero outside the grid.  Each row of the tile
   // is read through a buffer resource over its in-grid part (none for rows outside the
   // grid): the range check returns 0 for the halo cells, so there are no clamps or
-  // selects, and the lane offsets are the same for every row.  Rows r = w + 4 k (k < 25)
-  // of the 10 x 10 rows; element e = lane of each row, and e = 64 + lane for lanes 0, 1.
+  // selects, and the lane offsets are the same for every row.  Rows r = w + 4 k
+  // (k < kRows) of the (FI + 2) x 10 rows; element e = lane of each row, and e = 64 + lane
+  // for lanes 0, 1.
   const TX* __restrict__ x = (const TX*)a.x;
   const f16* __restrict__ img = (const f16*)a.img;
-  static_assert(kPi * kPj == 100 && kPz == 66, "staging assumes 100 rows of 66 cells");
+  static_assert((FI + 2) * kPj % 4 == 0 && kPz == 66, "staging assumes 4 waves x kRows rows of 66 cells");
   int zb[2];  // byte offsets of the lane's cells in a row (negative wraps: out of range)
 #pragma unroll
   for (int u = 0; u < 2; ++u) zb[u] = (z0 - 1 + (u == 0 ? lane : 64 + (lane & 1))) * (int)sizeof(TX);
 #pragma unroll
-  for (int k0 = 0; k0 < 25; k0 += 13) {
+  for (int k0 = 0; k0 < kRows; k0 += 13) {
     constexpr int KB = 13;
     bf16 xv[KB][2];
     f16 iv[KB][C][2];
 #pragma unroll
     for (int kk = 0; kk < KB; ++kk) {
       const int r = w + 4 * (k0 + kk);
-      if (k0 + kk < 25) {
+      if (k0 + kk < kRows) {
         const int pi = r / kPj, pj = r - pi * kPj;
         const int gi = i0 + pi - 1, gj = j0 + pj - 1;
         const bool rok = gi >= 0 && gi < a.n1 && gj >= 0 && gj < a.n2;
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
 #pragma unroll
     for (int kk = 0; kk < KB; ++kk) {
       const int r = w + 4 * (k0 + kk);
-      if (k0 + kk < 25) {
+      if (k0 + kk < kRows) {
         xs[r * kPz + lane] = xv[kk][0];
 #pragma unroll
         for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + lane] = iv[kk][c][0];
@@ -298,11 +302,11 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   uint8_t ka = lane_at(mA + vr, mob), kb = lane_at(mB + vr, mob);
   TP pv = lane_at(pP + vr, mop);
   float da = dots ? lane_at(d0p + vr, mof) : 0.f, db = dots ? lane_at(d1p + vr * rstep1, d1off) : 0.f;
-  for (int q = 0; q < kFi * 2; ++q) {
+  for (int q = 0; q < FI * 2; ++q) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
     const int cb = ((ii + 1) * kPj + (jj + 1)) * kPz + 1;
     // next chunk's epilogue operands
-    const long vn = row(q + 1 < kFi * 2 ? q + 1 : q);
+    const long vn = row(q + 1 < FI * 2 ? q + 1 : q);
     const uint8_t ka_n = lane_at(mA + vn, mob), kb_n = lane_at(mB + vn, mob);
     const TP pv_n = lane_at(pP + vn, mop);
     const float da_n = dots ? lane_at(d0p + vn, mof) : 0.f;
@@ -375,10 +379,21 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   if (dots) block_partials<2>(red, a.partials);
 }
 
-unsigned fused_blocks(int n1, int n2, int n3) { return (unsigned)((n1 / kFi) * (n2 / kFj) * (n3 / kFz)); }
+// Tile depth along i: 8 planes, or 4 where 8-plane tiles would leave fewer than
+// kMinFusedBlocks workgroups (about eight per CU) to spread over the chip.
+constexpr unsigned kMinFusedBlocks = 2048;
+
+static int fused_fi(int n1, int n2, int n3) {
+  const unsigned b8 = (unsigned)((n1 / 8) * (n2 / kFj) * (n3 / kFz));
+  return b8 < kMinFusedBlocks ? 4 : 8;
+}
+
+unsigned fused_blocks(int n1, int n2, int n3) {
+  return (unsigned)((n1 / fused_fi(n1, n2, n3)) * (n2 / kFj) * (n3 / kFz));
+}
 
 bool fused_eligible(int C, int B, int n1, int n2, int n3) {
-  return (C == 1 || C == 2) && B == 1 && n1 % kFi == 0 && n2 % kFj == 0 && n3 % kFz == 0 && n3 < (1 << 24);
+  return (C == 1 || C == 2) && B == 1 && n1 % 8 == 0 && n2 % kFj == 0 && n3 % kFz == 0 && n3 < (1 << 24);
 }
 
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
@@ -390,8 +405,9 @@ int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, h
 
 template <typename TX, typename TY, int C>
 static void launch_fused_t(const FusedArgs& a, hipStream_t st) {
-  const unsigned nb = (unsigned)((a.n1 / kFi) * (a.n2 / kFj) * (a.n3 / kFz));
-  sm_fused_kernel<TX, TY, bf16, C><<<nb, kThreads, 0, st>>>(a);
+  const unsigned nb = fused_blocks(a.n1, a.n2, a.n3);
+  if (fused_fi(a.n1, a.n2, a.n3) == 8) sm_fused_kernel<TX, TY, bf16, C, 8><<<nb, kThreads, 0, st>>>(a);
+  else sm_fused_kernel<TX, TY, bf16, C, 4><<<nb, kThreads, 0, st>>>(a);
 }
 
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& args, hipStream_t st) {

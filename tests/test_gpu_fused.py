@@ -71,6 +71,7 @@ def _nrel(a, b):
     (1, False, 2, (24, 40, 192)),   # odd tile counts; level 1 (12, 20, 96) stays on stored K
     (2, True, 1, (16, 8, 64)),      # two channels, model mask
     (2, False, 2, (32, 16, 128)),
+    (1, False, 1, (256, 128, 256)),  # 2048 tiles: 8-plane tiles on level 0, 4-plane on level 1
 ])
 def test_fused_matches_stored_stencils(dev, C, mask, n, shape):
     """With exactly representable weights, images and input, both BF16 plans compute the

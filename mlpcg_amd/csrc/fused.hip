@@ -298,6 +298,38 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
     return ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0;
   };
+  // the LDS words one half-chunk (h) of chunk cell cb needs: per image channel p0, p2, q0,
+  // q2, the column-8 words of the even and odd groups; the x words of the two groups
+  struct Words {
+    uint32_t im[NK][6];
+    uint32_t xw[2][5];
+  };
+  auto cb_of = [&](int q) { return (((q >> 1) + 1) * kPj + (w + 4 * (q & 1) + 1)) * kPz + 1; };
+  auto load_words = [&](int cbq, int h) {
+    Words r;
+#pragma unroll
+    for (int kc = 0; kc < NK; ++kc) {
+      const f16* ib = is + kc * kTile + cbq + 32 * h;
+      r.im[kc][0] = lds_word(ib, iP);
+      r.im[kc][1] = lds_word(ib, iP + 2);
+      r.im[kc][2] = lds_word(ib, iQ);
+      r.im[kc][3] = lds_word(ib, iQ + 2);
+      r.im[kc][4] = lds_word(ib, i8e);
+      r.im[kc][5] = lds_word(ib, i8o);
+    }
+    const bf16* xb = xs + cbq + 32 * h;
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const int tt = 2 * h + par;
+      r.xw[par][0] = lds_word(xb, xP[tt]);
+      r.xw[par][1] = lds_word(xb, xP[tt] + 2);
+      r.xw[par][2] = lds_word(xb, xQ[tt]);
+      r.xw[par][3] = lds_word(xb, xQ[tt] + 2);
+      r.xw[par][4] = lds_word(xb, x8[tt]);
+    }
+    return r;
+  };
+  Words cur = load_words(cb_of(0), 0);
   long vr = row(0);
   uint8_t ka = lane_at(mA + vr, mob), kb = lane_at(mB + vr, mob);
   TP pv = lane_at(pP + vr, mop);
@@ -314,20 +346,19 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     f32x4 c2[4];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+      // LDS words of the next half-chunk ((q, 1) or (q + 1, 0)) are read before this
+      // half's products, so their latency overlaps the matrix-core work
+      const int qn = h == 0 ? q : (q + 1 < FI * 2 ? q + 1 : q);
+      const Words nxt = load_words(cb_of(qn), h ^ 1);
       // image words shared by the even (t = 2h) and odd (t = 2h + 1) groups of this half
       h8 bv[2][NK];
 #pragma unroll
       for (int kc = 0; kc < NK; ++kc) {
-        const f16* ib = is + kc * kTile + cb + 32 * h;
-        const uint32_t p0 = lds_word(ib, iP), p2 = lds_word(ib, iP + 2);
-        const uint32_t q0 = lds_word(ib, iQ), q2 = lds_word(ib, iQ + 2);
-        uint32_t e8 = lds_word(ib, i8e), d8 = lds_word(ib, i8o);
-        if (kc == 0) {
-          e8 = bias_lane ? kOnesF16 : e8;
-          d8 = bias_lane ? kOnesF16 : d8;
-        }
-        bv[0][kc] = __builtin_bit_cast(h8, u32x4{p0, pack_lo(p2, q2), q0, e8});
-        bv[1][kc] = __builtin_bit_cast(h8, u32x4{p2, pack_hi(p0, q0), q2, d8});
+        const uint32_t* m = cur.im[kc];
+        const uint32_t e8 = (kc == 0 && bias_lane) ? kOnesF16 : m[4];
+        const uint32_t d8 = (kc == 0 && bias_lane) ? kOnesF16 : m[5];
+        bv[0][kc] = __builtin_bit_cast(h8, u32x4{m[0], pack_lo(m[1], m[3]), m[2], e8});
+        bv[1][kc] = __builtin_bit_cast(h8, u32x4{m[1], pack_hi(m[0], m[2]), m[3], d8});
       }
 #pragma unroll
       for (int par = 0; par < 2; ++par) {
@@ -338,12 +369,9 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
           d0 = Mfma<f16>::run(A[par][0][kc], bv[par][kc], d0);
           d1 = Mfma<f16>::run(A[par][1][kc], bv[par][kc], d1);
         }
-        const bf16* xb = xs + cb + 32 * h;
-        const uint32_t p0 = lds_word(xb, xP[tt]), p2 = lds_word(xb, xP[tt] + 2);
-        const uint32_t q0 = lds_word(xb, xQ[tt]), q2 = lds_word(xb, xQ[tt] + 2);
-        const uint32_t w8 = lds_word(xb, x8[tt]);
-        const v8 ax = par == 0 ? __builtin_bit_cast(v8, u32x4{p0, pack_lo(p2, q2), q0, w8})
-                               : __builtin_bit_cast(v8, u32x4{p2, pack_hi(p0, q0), q2, w8});
+        const uint32_t* xw = cur.xw[par];
+        const v8 ax = par == 0 ? __builtin_bit_cast(v8, u32x4{xw[0], pack_lo(xw[1], xw[3]), xw[2], xw[4]})
+                               : __builtin_bit_cast(v8, u32x4{xw[1], pack_hi(xw[0], xw[2]), xw[3], xw[4]});
         v8 bk;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -352,6 +380,7 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
         }
         c2[tt] = Mfma<bf16>::run(ax, bk, zero4);
       }
+      cur = nxt;
     }
     // this lane's voxel: group tstar, element nn % 4 (selects, no branches)
     f32x4 cs = c2[0];

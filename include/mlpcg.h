@@ -174,7 +174,7 @@ void mlpcg_precond_destroy(mlpcg_precond* P);
  * the fly): for BF16 plans created after this call, blocks on grids with
  * n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 2, not transposed) recompute K on the matrix
  * cores inside the apply instead of storing it.  mode 0: never (K stored like every other
- * block); 1 (the default): on such grids of at least 2^20 voxels (smaller ones are faster
+ * block); 1 (the default): on such grids of at least 2^18 voxels (smaller ones are faster
  * with stored K); 2: on every such grid.  Process-wide; returns the previous mode. */
 int mlpcg_set_fused_apply(int mode);
 /* No counterpart in the reference: SmallSM plans (swap off, FP32/FP16/BF16) created after

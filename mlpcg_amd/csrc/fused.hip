@@ -19,8 +19,8 @@
 // significant bits), which makes K more accurate than a FP32-computed K rounded to BF16;
 // step 2 rounds x and K to BF16 (the storage type of the plan) and accumulates in FP32.
 //
-// Tile: a block of 4 waves covers FI planes x 8 rows x 64 z-voxels (FI = 8, or 4 where
-// 8-plane tiles give too few blocks); x and img of the tile plus a one-cell halo are
+// Tile: a block of 4 waves covers FI planes x 8 rows x 64 z-voxels (FI = 8, or 4 / 2
+// where 8-plane tiles give too few blocks); x and img of the tile plus a one-cell halo are
 // staged in LDS; a wave processes 64 z-voxels of one (plane, row) per step: 4 groups of
 // 16 voxels.
 #include "common.hpp"
@@ -30,7 +30,7 @@ namespace mlpcg {
 
 namespace {
 
-constexpr int kFz = 64, kFj = 8;  // tile: FI planes (8, or 4 on small grids) x 8 rows x 64 z
+constexpr int kFz = 64, kFj = 8;  // tile: FI planes (8, or 4 / 2 on small grids) x 8 rows x 64 z
 constexpr int kPz = kFz + 2, kPj = kFj + 2;
 __host__ __device__ constexpr int tile_cells(int FI) { return (FI + 2) * kPj * kPz; }
 
@@ -408,13 +408,14 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   if (dots) block_partials<2>(red, a.partials);
 }
 
-// Tile depth along i: 8 planes, or 4 where 8-plane tiles would leave fewer than
-// kMinFusedBlocks workgroups (about eight per CU) to spread over the chip.
+// Tile depth along i: 8 planes; 4 where 8-plane tiles would leave fewer than
+// kMinFusedBlocks workgroups (about eight per CU) to spread over the chip; 2 where even
+// 4-plane tiles give fewer than a quarter of that.
 constexpr unsigned kMinFusedBlocks = 2048;
 
 static int fused_fi(int n1, int n2, int n3) {
   const unsigned b8 = (unsigned)((n1 / 8) * (n2 / kFj) * (n3 / kFz));
-  return b8 < kMinFusedBlocks ? 4 : 8;
+  return b8 >= kMinFusedBlocks ? 8 : (2 * b8 >= kMinFusedBlocks / 4 ? 4 : 2);
 }
 
 unsigned fused_blocks(int n1, int n2, int n3) {
@@ -435,8 +436,10 @@ int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, h
 template <typename TX, typename TY, int C>
 static void launch_fused_t(const FusedArgs& a, hipStream_t st) {
   const unsigned nb = fused_blocks(a.n1, a.n2, a.n3);
-  if (fused_fi(a.n1, a.n2, a.n3) == 8) sm_fused_kernel<TX, TY, bf16, C, 8><<<nb, kThreads, 0, st>>>(a);
-  else sm_fused_kernel<TX, TY, bf16, C, 4><<<nb, kThreads, 0, st>>>(a);
+  const int fi = fused_fi(a.n1, a.n2, a.n3);
+  if (fi == 8) sm_fused_kernel<TX, TY, bf16, C, 8><<<nb, kThreads, 0, st>>>(a);
+  else if (fi == 4) sm_fused_kernel<TX, TY, bf16, C, 4><<<nb, kThreads, 0, st>>>(a);
+  else sm_fused_kernel<TX, TY, bf16, C, 2><<<nb, kThreads, 0, st>>>(a);
 }
 
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& args, hipStream_t st) {

@@ -23,11 +23,11 @@ static size_t dsize(int dt) {
   }
 }
 
-// 0 off, 1 grids of at least kFusedMinVoxels (below that the 8x8x64 tiles are too few to
-// fill the GPU and the stored-stencil apply is faster: 64^3 runs 18 us fused, 10 us stored),
-// 2 every eligible grid
+// 0 off, 1 grids of at least kFusedMinVoxels (below that even 2-plane tiles are too few
+// to fill the GPU; at 64^3 with 2-plane tiles the fused apply already beats the stored
+// stencils: a 256^3 solve runs 0.75 ms faster with level 2 fused), 2 every eligible grid
 static int g_fused_apply = 1;
-constexpr long kFusedMinVoxels = 1L << 20;
+constexpr long kFusedMinVoxels = 1L << 18;
 void set_fused_apply(int mode) { g_fused_apply = (mode < 0 || mode > 2) ? 1 : mode; }
 int fused_apply_enabled() { return g_fused_apply; }
 // off by default: one workgroup streams the coarse stencils too slowly (85 us at 256^3,

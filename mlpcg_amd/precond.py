@@ -25,7 +25,7 @@ def set_fused_apply(mode) -> int:
     """Process-wide policy for plans created afterwards: BF16 blocks on grids with
     n1, n2 % 8 == 0 and n3 % 64 == 0 recompute their stencils on the matrix cores inside
     the apply instead of storing them.  mode 0 / False: never; 1 / True (default): on such
-    grids of at least 2^20 voxels; 2: on every such grid.  Returns the previous mode."""
+    grids of at least 2^18 voxels; 2: on every such grid.  Returns the previous mode."""
     return int(lib.mlpcg_set_fused_apply(int(mode)))
 
 

@@ -277,7 +277,8 @@ def workload_spheres128(args, dev):
 
 def workload_train256(args, dev):
     """Training at 256^3 (4 scenes, 40 spheres r 4-30, air fraction 20-50 %): BF16
-    forward/backward with FP32 master weights vs FP32, stencils recomputed in backward;
+    forward/backward with FP32 master weights vs FP32, stencils recomputed in backward,
+    each Adam step replayed as one HIP graph (fresh right-hand sides every step);
     steps/s, peak HBM and PCG iterations on 8 held-out 256^3 scenes after the same
     number of steps from the same starting weights."""
     import torch
@@ -291,18 +292,20 @@ def workload_train256(args, dev):
     for act_name, act in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
         model, wsrc = load_model(args.weights, args.levels, dev)
         model.train()
-        opt = torch.optim.Adam(model.parameters(), lr=3e-4)
+        opt = torch.optim.Adam(model.parameters(), lr=3e-4, capturable=True)
         torch.cuda.reset_peak_memory_stats(dev)
         rhs = [T.white_rhs(sb, 1, gen) for sb in scenes]
-        T.train_step(model, scenes, rhs, opt, act_dtype=act)
+        step = T.GraphedTrainStep(model, scenes, rhs, opt, act_dtype=act)  # one HIP graph per step
+        sync()
         t0 = time.perf_counter()
         for k in range(args.train_steps):
             rhs = [T.white_rhs(sb, 1, gen) for sb in scenes]
-            loss = T.train_step(model, scenes, rhs, opt, act_dtype=act)
+            loss_t = step(rhs)
             if k % 25 == 0:
-                log(f"[bench] train256 {act_name} step {k} loss {loss:.4f}")
+                log(f"[bench] train256 {act_name} step {k} loss {float(loss_t):.4f}")
         sync()
         sps = args.train_steps / (time.perf_counter() - t0)
+        loss = float(step.loss)
         peak = torch.cuda.max_memory_allocated(dev) / 2 ** 30
         its = []
         model.eval()

@@ -228,8 +228,37 @@ def upsample(x, mode="trilinear", scale=1.0):
     return K.upsample2(x.contiguous(), mode, scale)
 
 
+class _CombFunction(torch.autograd.Function):
+    """c0 * x0 + c1 * x1 with coefficients of shape (1,) (sm_model_3d.py:245,258,282,286).
+    The coefficient gradients are dot products over the whole level; for FP32/BF16 they go
+    through the library's deterministic FP64 dot, so a training step that contains them
+    can be captured in a HIP graph (a torch reduction over many workgroups records memset
+    nodes, which give wrong results on replays after the first)."""
+
+    @staticmethod
+    def forward(ctx, c0, x0, c1, x1):
+        ctx.save_for_backward(c0, x0, c1, x1)
+        return c0.to(x0.dtype) * x0 + c1.to(x1.dtype) * x1
+
+    @staticmethod
+    def backward(ctx, g):
+        c0, x0, c1, x1 = ctx.saved_tensors
+        need = ctx.needs_input_grad
+
+        def coef_grad(x, c):
+            if x.dtype == torch.float64:
+                return (g.double() * x).sum().reshape(c.shape).to(c.dtype)
+            gf = g.float().contiguous().reshape(-1)
+            return K.dot(gf, x.float().contiguous().reshape(-1)).reshape(c.shape).to(c.dtype)
+
+        return (coef_grad(x0, c0) if need[0] else None,
+                c0.to(g.dtype) * g if need[1] else None,
+                coef_grad(x1, c1) if need[2] else None,
+                c1.to(g.dtype) * g if need[3] else None)
+
+
 def _comb(c0, x0, c1, x1):
-    return c0.to(x0.dtype) * x0 + c1.to(x1.dtype) * x1
+    return _CombFunction.apply(c0, x0, c1, x1)
 
 
 def _sm_image(image, x):

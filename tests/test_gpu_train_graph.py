@@ -63,6 +63,39 @@ def test_graphed_step_matches_eager_steps(dev, act):
     assert abs(le - float(graphed(steps[0]))) <= 1e-5 * max(1.0, abs(le))
 
 
+def test_each_replay_matches_an_eager_step_64(dev):
+    """64^3 scenes (level-0 reductions span many workgroups): before every replay the
+    graphed model's parameters and Adam state are copied into an eager twin, which takes
+    one eager step on the same right-hand sides; all parameters must agree after each
+    step, including the level-combine coefficients whose gradients are full-grid dots."""
+    from mlpcg_amd import train as T
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    torch.manual_seed(5)
+    mg = SmallSMModelDn3D(4, 1).to(dev)
+    me = copy.deepcopy(mg)
+    sbs = [T.SceneBatch(S.scene_spheres(64, 20 + i, num_spheres=3, rmin=4, rmax=10), 1, dev) for i in range(2)]
+    gen = torch.Generator(device=dev).manual_seed(6)
+    seq = [[T.white_rhs(sb, 1, gen) for sb in sbs] for _ in range(5)]
+    og = torch.optim.Adam(mg.parameters(), lr=3e-3, capturable=True)
+    oe = torch.optim.Adam(me.parameters(), lr=3e-3, capturable=True)
+    step = T.GraphedTrainStep(mg, sbs, seq[0], og)
+    names = [n for n, _ in mg.named_parameters()]
+    for k, rhs in enumerate(seq):
+        with torch.no_grad():
+            for p, q in zip(me.parameters(), mg.parameters()):
+                p.copy_(q)
+        oe.state.clear()
+        for pe, pg in zip(me.parameters(), mg.parameters()):
+            if pg in og.state:
+                oe.state[pe] = {kk: (v.clone() if torch.is_tensor(v) else v) for kk, v in og.state[pg].items()}
+        le = T.train_step(me, sbs, rhs, oe, loss_fn=T.relative_residual_sq_dot)
+        lg = float(step(rhs))
+        torch.cuda.synchronize()
+        assert abs(le - lg) <= 1e-6 * max(1.0, abs(le)), (k, le, lg)
+        for n, p, q in zip(names, me.parameters(), mg.parameters()):
+            assert torch.allclose(p, q, rtol=1e-6, atol=1e-7), (k, n, float((p - q).abs().max()))
+
+
 def test_graphed_steps_reduce_loss(dev):
     T, model, sbs, gen, B = _setup(dev)
     opt = torch.optim.Adam(model.parameters(), lr=3e-3, capturable=True)

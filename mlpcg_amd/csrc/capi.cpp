@@ -95,9 +95,11 @@ static int block_bwd(bool trans, const void* g, const void* image, const void* x
   MLPCG_REQUIRE(aligned16(g) && aligned16(x) && aligned16(gx) && aligned16(workspace) && aligned16(image),
                 "%s: pointers must be 16-byte aligned", who);
   const size_t V = (size_t)n1 * n2 * n3;
-  // the input gradient is the opposite stencil: transpose for the block, block for the
-  // transpose (stencil weights recomputed here, not saved from the forward pass)
-  if (launch_stencil_weights(dtype, image, weight, bias, workspace, C, n1, n2, n3, !trans, STREAM(stream)))
+  // the input gradient is the adjoint apply, both from the forward stencils K (recomputed
+  // here, not saved from the forward pass): for the block y = sum_p K_p(v) x(v + d_p) it
+  // is the transposed gather sum_p K_p(v - d_p) g(v - d_p); for the transpose block it is
+  // the block apply itself
+  if (launch_stencil_weights(dtype, image, weight, bias, workspace, C, n1, n2, n3, false, STREAM(stream)))
     return -1;
   ApplyArgs a;
   memset(&a, 0, sizeof(a));
@@ -108,7 +110,11 @@ static int block_bwd(bool trans, const void* g, const void* image, const void* x
   a.n1 = n1;
   a.n2 = n2;
   a.n3 = n3;
-  if (launch_sm_apply(dtype, dtype, dtype, a, STREAM(stream))) return -1;
+  if (trans) {
+    if (launch_sm_apply(dtype, dtype, dtype, a, STREAM(stream))) return -1;
+  } else {
+    if (launch_sm_apply_transposed(dtype, a, STREAM(stream))) return -1;
+  }
   void* part = (char*)workspace + ((27 * V * elem_size(dtype) + 255) & ~(size_t)255);
   // block: G_p(v) = sum_b g[b,v] x[b,v+d_p]; transpose: H_p(u) = sum_b x[b,u] g[b,u+d_p]
   const void* av = trans ? x : g;

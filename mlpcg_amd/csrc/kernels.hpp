@@ -87,6 +87,9 @@ int launch_coarse_tail(int dtype, const TailArgs& a, hipStream_t st);
 int launch_stencil_weights(int dtype, const void* img, const void* W, const void* b, void* K, int C,
                            int n1, int n2, int n3, bool trans, hipStream_t st);
 int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& a, hipStream_t st);
+// y[b, v] = sum_p K_p(v - d_p) x[b, v - d_p] from the forward stencils K (K, x, y of type
+// tk; uses a.K, a.x, a.y, a.B and the grid only)
+int launch_sm_apply_transposed(int tk, const ApplyArgs& a, hipStream_t st);
 int launch_avgpool2(int dtype, const void* in, void* out, int B, int n1, int n2, int n3,
                     const int* done, hipStream_t st);
 int launch_upsample2(int dtype, const UpArgs& a, hipStream_t st);

@@ -171,6 +171,56 @@ __global__ __launch_bounds__(kThreads) void stencil_weights_t_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// sm_apply_t: the transpose of a stored-stencil apply,
+//   y[b, v] = sum_p K_p(v - d_p) x[b, v - d_p]       (sm_block_trans_3d_kernel.cu:46-73)
+// gathered from the forward stencils K (no transposed stencils needed): for the source
+// row (i - d1, j - d2) the K rows of its three taps and the x row are read over
+// [k0 - 1, k0 + VZ] (zero outside the grid), and tap d3 contributes at offset -d3.
+// ---------------------------------------------------------------------------
+template <typename TK, typename TX, int VZ, int BB>
+__global__ __launch_bounds__(kThreads) void sm_apply_t_kernel(ApplyArgs a) {
+  typedef typename AccOf<TK>::type A;
+  ChunkPos c;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (!chunk_pos<VZ>(cid, a.n1, a.n2, a.n3, c)) return;
+  const long V = (long)a.n1 * a.n2 * a.n3;
+  const int b0 = blockIdx.y * BB;
+  const TK* __restrict__ K = (const TK*)a.K;
+  const TX* __restrict__ x = (const TX*)a.x;
+  A acc[BB][VZ];
+#pragma unroll
+  for (int bb = 0; bb < BB; ++bb)
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) acc[bb][t] = A(0);
+#pragma unroll
+  for (int a1 = 0; a1 < 3; ++a1) {
+#pragma unroll
+    for (int a2 = 0; a2 < 3; ++a2) {
+      const int ri = c.i + 1 - a1, rj = c.j + 1 - a2;  // source row (i - d1, j - d2)
+      A kh[3][VZ + 2];
+#pragma unroll
+      for (int a3 = 0; a3 < 3; ++a3)
+        load_row<VZ>(K + (long)(9 * a1 + 3 * a2 + a3) * V, ri, rj, c.k0, a.n1, a.n2, a.n3, kh[a3]);
+#pragma unroll
+      for (int bb = 0; bb < BB; ++bb) {
+        if (BB > 1 && b0 + bb >= a.B) break;
+        A row[VZ + 2];
+        load_row<VZ>(x + (long)(b0 + bb) * V, ri, rj, c.k0, a.n1, a.n2, a.n3, row);
+        // tap a3 (d3 = a3 - 1) reads source z = k0 + t - d3: halo index t + 2 - a3
+#pragma unroll
+        for (int t = 0; t < VZ; ++t)
+          acc[bb][t] += kh[0][t + 2] * row[t + 2] + kh[1][t + 1] * row[t + 1] + kh[2][t] * row[t];
+      }
+    }
+  }
+#pragma unroll
+  for (int bb = 0; bb < BB; ++bb) {
+    if (BB > 1 && b0 + bb >= a.B) break;
+    store_vec<VZ>((TX*)a.y + (long)(b0 + bb) * V + c.v0, acc[bb]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // sm_apply
 // ---------------------------------------------------------------------------
 template <typename TK, typename TX, typename TY, typename TP, int VZ, int BB>
@@ -523,6 +573,35 @@ static void launch_apply_t(const ApplyArgs& a, int vz, hipStream_t st) {
   MLPCG_AP(8, 1) MLPCG_AP(4, 1) MLPCG_AP(2, 1) MLPCG_AP(1, 1)
   MLPCG_AP(8, 4) MLPCG_AP(4, 4) MLPCG_AP(2, 4) MLPCG_AP(1, 4)
 #undef MLPCG_AP
+}
+
+template <typename TK>
+static void launch_apply_tr_t(const ApplyArgs& a, int vz, hipStream_t st) {
+  const long items = (long)a.n1 * a.n2 * (a.n3 / vz);
+  const int bb = a.B == 1 ? 1 : 4;
+  const int gy = (a.B + bb - 1) / bb;
+#define MLPCG_AT(VZZ, BBB)                                                                 \
+  if (vz == VZZ && bb == BBB) {                                                            \
+    sm_apply_t_kernel<TK, TK, VZZ, BBB><<<grid1(items, gy), kThreads, 0, st>>>(a);        \
+    return;                                                                                \
+  }
+  MLPCG_AT(8, 1) MLPCG_AT(4, 1) MLPCG_AT(2, 1) MLPCG_AT(1, 1)
+  MLPCG_AT(8, 4) MLPCG_AT(4, 4) MLPCG_AT(2, 4) MLPCG_AT(1, 4)
+#undef MLPCG_AT
+}
+
+int launch_sm_apply_transposed(int tk, const ApplyArgs& args, hipStream_t st) {
+  MLPCG_REQUIRE(args.B >= 1 && args.n1 >= 1 && args.n2 >= 1 && args.n3 >= 1, "sm_apply_t: bad sizes");
+  const int vz = pick_vz_small(args.n3, tk == kF64 ? 2 : (tk == kF32 ? 4 : 8),
+                               (long)args.n1 * args.n2 * args.n3 * args.B);
+  switch (tk) {
+    case kF64: launch_apply_tr_t<double>(args, vz, st); break;
+    case kF32: launch_apply_tr_t<float>(args, vz, st); break;
+    case kF16: launch_apply_tr_t<f16>(args, vz, st); break;
+    case kBF16: launch_apply_tr_t<bf16>(args, vz, st); break;
+    default: MLPCG_REQUIRE(false, "sm_apply_t: bad dtype %d", tk);
+  }
+  return check_launch("sm_apply_t");
 }
 
 int launch_sm_apply(int tk, int tx, int ty, const ApplyArgs& args, hipStream_t st) {

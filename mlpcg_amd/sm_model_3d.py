@@ -357,15 +357,24 @@ class SmallSMModelDn3D(BaseModel):
                 imgs.append(K.avgpool2(imgs[-1]))
             ybars = [K.sm_linear(imgs[i], self.c0[i].weight, self.c0[i].bias)[1]
                      for i in range(self.n)]
-        return imgs, ybars
+            # rows of the c0 (levels 0..n-1) then c1 blocks: ybar of their level
+            Y = torch.stack([y.reshape(-1) for y in ybars + ybars])
+        return imgs, ybars, Y
 
     def forward_cached(self, cache, b):
         """forward(image, b) with the image pyramid and mean patches of image_cache: same
-        result (c = w . ybar + bias by torch, whose gradients are smlinear3d.backward's)."""
-        imgs, ybars = cache
+        result.  All 2n coefficients c = w . ybar + bias come from one batched product
+        (its gradients are smlinear3d.backward's: gw = g ybar, gb = g)."""
+        imgs, ybars, Y = cache
+        lins = list(self.c0) + list(self.c1)
+        Wm = torch.stack([m.weight.reshape(-1) for m in lins]).to(Y.dtype)
+        bv = torch.cat([m.bias for m in lins]).to(Y.dtype)
+        coef = (Wm * Y).sum(1) + bv
+        n = self.n
 
-        def lin(m, ybar):
-            return (m.weight.reshape(-1).to(ybar.dtype) * ybar.reshape(-1)).sum().reshape(1) + m.bias.to(ybar.dtype)
+        def lin(which, i):
+            k = which * n + i
+            return coef[k:k + 1]
 
         x = [self.pre[0](imgs[0], b)]
         for i in range(1, self.n):
@@ -375,8 +384,8 @@ class SmallSMModelDn3D(BaseModel):
         x[-1] = self.l(imgs[self.n], x[-1])
         for i in range(self.n, 0, -1):
             x[i] = upsample(x[i], self.mode)
-            c0 = lin(self.c0[i - 1], ybars[i - 1])
-            c1 = lin(self.c1[i - 1], ybars[i - 1])
+            c0 = lin(0, i - 1)
+            c1 = lin(1, i - 1)
             if self.swap_ord:
                 x[i - 1] = _comb(c0, x[i - 1], c1, x[i])
                 x[i - 1] = self.post[i - 1](imgs[i - 1], x[i - 1])

@@ -278,6 +278,98 @@ __global__ __launch_bounds__(kThreads) void weight_grad_partial_kernel(
   }
 }
 
+// One-channel weight gradient on the matrix cores (FP32 MFMA, so FP32 products and sums
+// as in the VALU kernel above): gW[p][q] = sum_v G_p(v) P_q(v) with P_q the 27 image taps
+// and the bias column (P_27 = 1), a 32 x 32 (padded) product over the voxels.  A wave
+// walks one z-row segment of up to 64 voxels, 4 voxels per v_mfma_f32_16x16x4_f32
+// (A[i][k]: lane 16k + i; B[k][j]: lane 16k + j; D[4(l/16)+t][l%16]): lane (k, i) feeds
+// G and P for taps i and 16 + i of voxel k.  Its neighbour offsets are constants and the
+// validity of its two neighbour rows is fixed per segment, so only the z ends need a
+// check.  Waves of a block add their 32 x 32 tiles in LDS in a fixed order; the block's
+// 27 x 28 sums go to partials[block] like the VALU kernel's.
+typedef float wg_f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename TX, typename TI>
+__global__ __launch_bounds__(kThreads) void weight_grad_mfma_kernel(
+    const TX* __restrict__ a, const TX* __restrict__ cv, int B, const TI* __restrict__ img, int n1,
+    int n2, int n3, float* __restrict__ partials) {
+  constexpr int NQ = 28, NO = 27 * NQ;
+  __shared__ float red[4][32 * 32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int kq = lane >> 4, il = lane & 15;
+  const long V = (long)n1 * n2 * n3;
+  const long s1 = (long)n2 * n3;
+  // voxels per segment: the largest of 64, 32, 16, 8, 4 that divides n3 (n3 % 4 == 0)
+  const int seg = n3 % 64 == 0 ? 64 : (n3 % 32 == 0 ? 32 : (n3 % 16 == 0 ? 16 : (n3 % 8 == 0 ? 8 : 4)));
+  const int nseg3 = n3 / seg;
+  const long nseg = (long)n1 * n2 * nseg3;
+  // this lane's two taps (rows of G, columns of P): t = il and 16 + il
+  int d1[2], d2[2], d3[2];
+  bool gok[2], pimg[2], pone[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int t = 16 * h + il;
+    const int tt = t < 27 ? t : 0;
+    d1[h] = tt / 9 - 1;
+    d2[h] = (tt / 3) % 3 - 1;
+    d3[h] = tt % 3 - 1;
+    gok[h] = t < 27;
+    pimg[h] = t < 27;
+    pone[h] = t == 27;
+  }
+  wg_f32x4 acc[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int g = 0; g < 2; ++g) acc[h][g] = wg_f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long sg = (long)blockIdx.x * 4 + w; sg < nseg; sg += (long)gridDim.x * 4) {
+    const int zs = (int)(sg % nseg3) * seg;
+    const long rowid = sg / nseg3;
+    const int j = (int)(rowid % n2), i = (int)(rowid / n2);
+    const long vrow = rowid * n3;
+    bool rok[2];
+    long roff[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      rok[h] = (unsigned)(i + d1[h]) < (unsigned)n1 && (unsigned)(j + d2[h]) < (unsigned)n2;
+      roff[h] = (long)d1[h] * s1 + (long)d2[h] * n3 + d3[h];
+    }
+#pragma unroll 4
+    for (int z4 = 0; z4 < seg; z4 += 4) {
+      const int z = zs + z4 + kq;
+      const long v = vrow + z;
+      float gv[2] = {0.f, 0.f}, pv[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const bool ok = rok[h] && (unsigned)(z + d3[h]) < (unsigned)n3;
+        const long u = ok ? v + roff[h] : v;
+        float sacc = 0.f;
+        for (int b = 0; b < B; ++b) sacc += (float)a[(long)b * V + v] * (float)cv[(long)b * V + u];
+        gv[h] = (gok[h] && ok) ? sacc : 0.f;
+        const float iv = (float)img[u];
+        pv[h] = pone[h] ? 1.f : ((pimg[h] && ok) ? iv : 0.f);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int g = 0; g < 2; ++g) acc[h][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(gv[h], pv[g], acc[h][g], 0, 0, 0);
+    }
+  }
+  // D[16h + 4 kq + t][16g + il] of this wave -> LDS, then a fixed-order sum over the waves
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) red[w][(16 * h + 4 * kq + t) * 32 + 16 * g + il] = acc[h][g][t];
+  __syncthreads();
+  for (int o = threadIdx.x; o < NO; o += kThreads) {
+    const int pp = o / NQ, qq = o % NQ;
+    const int e = pp * 32 + qq;
+    partials[(size_t)blockIdx.x * NO + o] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+  }
+}
+
 template <typename A>
 __global__ __launch_bounds__(kThreads) void weight_grad_final_kernel(const A* __restrict__ partials,
                                                                      int nblk, int C,
@@ -335,6 +427,26 @@ int launch_weight_grad(int dtype, const void* a, const void* c, const void* img,
                        void* workspace, int C, int B, int n1, int n2, int n3, hipStream_t st) {
   MLPCG_REQUIRE(C >= 1 && C <= 3, "weight_grad: C must be 1, 2 or 3 (got %d)", C);
   const int nblk = wg_blocks(dtype, C, n1, n2, n3);
+  if (C == 1 && dtype != kF64 && n3 % 4 == 0) {
+    // matrix-core kernel (same partials layout and block count)
+    switch (dtype) {
+      case kF32:
+        weight_grad_mfma_kernel<float, float><<<nblk, kThreads, 0, st>>>(
+            (const float*)a, (const float*)c, B, (const float*)img, n1, n2, n3, (float*)workspace);
+        break;
+      case kF16:
+        weight_grad_mfma_kernel<f16, f16><<<nblk, kThreads, 0, st>>>(
+            (const f16*)a, (const f16*)c, B, (const f16*)img, n1, n2, n3, (float*)workspace);
+        break;
+      default:
+        weight_grad_mfma_kernel<bf16, bf16><<<nblk, kThreads, 0, st>>>(
+            (const bf16*)a, (const bf16*)c, B, (const bf16*)img, n1, n2, n3, (float*)workspace);
+        break;
+    }
+    weight_grad_final_kernel<float><<<27 * 28, kThreads, 0, st>>>((const float*)workspace, nblk, C,
+                                                                  (float*)gW, (float*)gb);
+    return check_launch("weight_grad");
+  }
   switch (dtype) {
     case kF64: launch_wg_t<double, double>(dtype, a, c, img, workspace, nblk, C, B, n1, n2, n3, st); break;
     case kF32: launch_wg_t<float, float>(dtype, a, c, img, workspace, nblk, C, B, n1, n2, n3, st); break;

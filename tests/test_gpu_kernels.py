@@ -137,6 +137,24 @@ def test_sm_block_backward_finite_differences_f64(dev, trans):
         assert abs(fd - float((gx * dx).sum())) <= 1e-6 * max(1.0, abs(fd))
 
 
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("B,shape", [(2, (6, 5, 8)), (1, (4, 6, 12)), (3, (3, 4, 20)), (1, (8, 8, 96)),
+                                     (2, (5, 4, 6))])
+def test_sm_block_backward_f32_vs_f64(dev, trans, B, shape):
+    """One-channel FP32 backward (matrix-core weight gradient where n3 % 4 == 0, VALU
+    otherwise) against the FP64 backward on the same inputs: dx, dW, db."""
+    from mlpcg_amd import kernels as K
+    img = _rand((1, *shape), 40).abs()
+    x, g = _rand((B, 1, *shape), 41), _rand((B, 1, *shape), 42)
+    W, b = _rand((27, 1, 3, 3, 3), 43) / 9, _rand((27,), 44) / 9
+    f = K.sm_block_trans_backward if trans else K.sm_block_backward
+    ref = f(g.to(dev), img.to(dev), x.to(dev), W.to(dev), b.to(dev))
+    out = f(g.float().to(dev), img.float().to(dev), x.float().to(dev), W.float().to(dev), b.float().to(dev))
+    for o, r in zip(out, ref):
+        assert o.dtype == torch.float32
+        assert _rel(o, r) < 2e-6
+
+
 @pytest.mark.parametrize("C,shape", [(1, (8, 8, 8)), (3, (6, 10, 12)), (2, (1, 4, 5)), (1, (16, 16, 32))])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_sm_linear(dev, C, shape, dtype):

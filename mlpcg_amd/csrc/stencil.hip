@@ -113,6 +113,43 @@ __global__ __launch_bounds__(kThreads) void stencil_weights_kernel(
   }
 }
 
+// The same stencils with one thread per (tap, voxel): on small grids the per-voxel kernel
+// above is a chain of 27 * 28 dependent FMAs per thread on a few waves; here each thread
+// computes one K value (same summation order, so the same result).
+template <typename TI, typename TK, int C>
+__global__ __launch_bounds__(kThreads) void stencil_weights_small_kernel(
+    const TI* __restrict__ img, const typename AccOf<TK>::type* __restrict__ W,
+    const typename AccOf<TK>::type* __restrict__ bias, TK* __restrict__ K, int n1, int n2, int n3) {
+  typedef typename AccOf<TK>::type A;
+  const long V = (long)n1 * n2 * n3;
+  const long idx = (long)blockIdx.x * kThreads + threadIdx.x;  // p * V + v
+  if (idx >= 27 * V) return;
+  const int p = (int)(idx / V);
+  const long v = idx - (long)p * V;
+  const int k = (int)(v % n3);
+  const long t = v / n3;
+  const int j = (int)(t % n2), i = (int)(t / n2);
+  const A* __restrict__ Wp = W + p * C * 27;
+  A acc = bias[p];
+#pragma unroll
+  for (int s = 0; s < C; ++s)
+#pragma unroll
+    for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+      for (int a2 = 0; a2 < 3; ++a2) {
+        const int ii = i + a1 - 1, jj = j + a2 - 1;
+        const bool rok = (unsigned)ii < (unsigned)n1 && (unsigned)jj < (unsigned)n2;
+        const long rb = (long)s * V + ((long)ii * n2 + jj) * n3;
+#pragma unroll
+        for (int a3 = 0; a3 < 3; ++a3) {
+          const int kk = k + a3 - 1;
+          const A xv = (rok && (unsigned)kk < (unsigned)n3) ? (A)img[rb + kk] : A(0);
+          acc += Wp[s * 27 + a1 * 9 + a2 * 3 + a3] * xv;
+        }
+      }
+  K[idx] = (TK)acc;
+}
+
 // Kt_q(v) = K_p(v - d_p) with p = 26 - q (so d_q = -d_p); zero where v + d_q is outside.
 template <typename TI, typename TK, int C, int VZ>
 __global__ __launch_bounds__(kThreads) void stencil_weights_t_kernel(
@@ -507,6 +544,9 @@ static inline const int* done_for(const int* done, long voxels) {
   return voxels >= kDoneCheckMinVoxels ? done : nullptr;
 }
 
+// grids up to this size compute their stencils with one thread per (tap, voxel)
+constexpr long kSmallWeightsVoxels = 1L << 18;
+
 static inline dim3 grid1(long items, int y = 1) {
   return dim3((unsigned)((items + kThreads - 1) / kThreads), (unsigned)y, 1);
 }
@@ -515,6 +555,20 @@ template <typename TI, typename TK>
 static void launch_weights_t(const void* img, const void* W, const void* b, void* K, int C, int n1,
                              int n2, int n3, bool trans, hipStream_t st) {
   typedef typename AccOf<TK>::type A;
+  const long V = (long)n1 * n2 * n3;
+  if (!trans && V <= kSmallWeightsVoxels) {
+    const long items = 27 * V;
+    if (C == 1)
+      stencil_weights_small_kernel<TI, TK, 1><<<grid1(items), kThreads, 0, st>>>(
+          (const TI*)img, (const A*)W, (const A*)b, (TK*)K, n1, n2, n3);
+    else if (C == 2)
+      stencil_weights_small_kernel<TI, TK, 2><<<grid1(items), kThreads, 0, st>>>(
+          (const TI*)img, (const A*)W, (const A*)b, (TK*)K, n1, n2, n3);
+    else
+      stencil_weights_small_kernel<TI, TK, 3><<<grid1(items), kThreads, 0, st>>>(
+          (const TI*)img, (const A*)W, (const A*)b, (TK*)K, n1, n2, n3);
+    return;
+  }
   const int maxvz = (C == 1) ? 4 : 2;
   const int vz = pick_vz(n3, maxvz);
   const long items = (long)n1 * n2 * (n3 / vz);

@@ -370,6 +370,93 @@ __global__ __launch_bounds__(kThreads) void weight_grad_mfma_kernel(
   }
 }
 
+// B == 1 form of weight_grad_mfma_kernel with the neighbour rows staged in LDS: per
+// z-row segment (one per wave; the block's four waves run the same number of rounds, so
+// the barriers are uniform) the 9 neighbour rows of x and of the image over
+// [z0 - 1, z0 + seg] are loaded with coalesced row reads, zero outside the grid, and the
+// matrix-core loop reads G and P from LDS without bounds checks.
+template <typename TX, typename TI>
+__global__ __launch_bounds__(kThreads) void weight_grad_mfma_lds_kernel(
+    const TX* __restrict__ a, const TX* __restrict__ cv, const TI* __restrict__ img, int n1, int n2,
+    int n3, float* __restrict__ partials) {
+  constexpr int NQ = 28, NO = 27 * NQ, RL = 66;  // staged row length (seg <= 64, + halo)
+  __shared__ float xs[4][9 * RL], is[4][9 * RL], ac[4][64];
+  __shared__ float red[4][32 * 32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int kq = lane >> 4, il = lane & 15;
+  const long s1 = (long)n2 * n3;
+  const int seg = n3 % 64 == 0 ? 64 : (n3 % 32 == 0 ? 32 : (n3 % 16 == 0 ? 16 : (n3 % 8 == 0 ? 8 : 4)));
+  const int nseg3 = n3 / seg;
+  const long nseg = (long)n1 * n2 * nseg3;
+  int rr[2], cz[2];  // this lane's taps t = il, 16 + il: staged row and z offset (t / 3, t % 3)
+  bool gok[2], pimg[2], pone[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int t = 16 * h + il;
+    const int tt = t < 27 ? t : 0;
+    rr[h] = tt / 3;
+    cz[h] = tt % 3;
+    gok[h] = t < 27;
+    pimg[h] = t < 27;
+    pone[h] = t == 27;
+  }
+  wg_f32x4 acc[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int g = 0; g < 2; ++g) acc[h][g] = wg_f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long sg0 = (long)blockIdx.x * 4; sg0 < nseg; sg0 += (long)gridDim.x * 4) {
+    const long sg = sg0 + w;
+    const bool active = sg < nseg;  // uniform per wave
+    if (active) {
+      const int zs = (int)(sg % nseg3) * seg;
+      const long rowid = sg / nseg3;
+      const int j = (int)(rowid % n2), i = (int)(rowid / n2);
+      for (int e = lane; e < 9 * (seg + 2); e += 64) {
+        const int r = e / (seg + 2), cc = e - r * (seg + 2);
+        const int ii = i + r / 3 - 1, jj = j + r % 3 - 1, z = zs - 1 + cc;
+        const bool ok = (unsigned)ii < (unsigned)n1 && (unsigned)jj < (unsigned)n2 && (unsigned)z < (unsigned)n3;
+        const long u = ok ? (long)ii * s1 + (long)jj * n3 + z : 0;
+        xs[w][r * RL + cc] = ok ? (float)cv[u] : 0.f;
+        is[w][r * RL + cc] = ok ? (float)img[u] : 0.f;
+      }
+      for (int e = lane; e < seg; e += 64) ac[w][e] = (float)a[rowid * n3 + zs + e];
+    }
+    __syncthreads();
+    if (active) {
+      for (int z4 = 0; z4 < seg; z4 += 4) {
+        const int z = z4 + kq;
+        const float av = ac[w][z];
+        float gv[2], pv[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int cidx = rr[h] * RL + z + cz[h];
+          gv[h] = gok[h] ? av * xs[w][cidx] : 0.f;
+          pv[h] = pone[h] ? 1.f : (pimg[h] ? is[w][cidx] : 0.f);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int g = 0; g < 2; ++g)
+            acc[h][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(gv[h], pv[g], acc[h][g], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) red[w][(16 * h + 4 * kq + t) * 32 + 16 * g + il] = acc[h][g][t];
+  __syncthreads();
+  for (int o = threadIdx.x; o < NO; o += kThreads) {
+    const int pp = o / NQ, qq = o % NQ;
+    const int e = pp * 32 + qq;
+    partials[(size_t)blockIdx.x * NO + o] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+  }
+}
+
 template <typename A>
 __global__ __launch_bounds__(kThreads) void weight_grad_final_kernel(const A* __restrict__ partials,
                                                                      int nblk, int C,
@@ -431,16 +518,24 @@ int launch_weight_grad(int dtype, const void* a, const void* c, const void* img,
     // matrix-core kernel (same partials layout and block count)
     switch (dtype) {
       case kF32:
-        weight_grad_mfma_kernel<float, float><<<nblk, kThreads, 0, st>>>(
-            (const float*)a, (const float*)c, B, (const float*)img, n1, n2, n3, (float*)workspace);
+        if (B == 1)
+          weight_grad_mfma_lds_kernel<float, float><<<nblk, kThreads, 0, st>>>(
+              (const float*)a, (const float*)c, (const float*)img, n1, n2, n3, (float*)workspace);
+        else
+          weight_grad_mfma_kernel<float, float><<<nblk, kThreads, 0, st>>>(
+              (const float*)a, (const float*)c, B, (const float*)img, n1, n2, n3, (float*)workspace);
         break;
       case kF16:
         weight_grad_mfma_kernel<f16, f16><<<nblk, kThreads, 0, st>>>(
             (const f16*)a, (const f16*)c, B, (const f16*)img, n1, n2, n3, (float*)workspace);
         break;
       default:
-        weight_grad_mfma_kernel<bf16, bf16><<<nblk, kThreads, 0, st>>>(
-            (const bf16*)a, (const bf16*)c, B, (const bf16*)img, n1, n2, n3, (float*)workspace);
+        if (B == 1)
+          weight_grad_mfma_lds_kernel<bf16, bf16><<<nblk, kThreads, 0, st>>>(
+              (const bf16*)a, (const bf16*)c, (const bf16*)img, n1, n2, n3, (float*)workspace);
+        else
+          weight_grad_mfma_kernel<bf16, bf16><<<nblk, kThreads, 0, st>>>(
+              (const bf16*)a, (const bf16*)c, B, (const bf16*)img, n1, n2, n3, (float*)workspace);
         break;
     }
     weight_grad_final_kernel<float><<<27 * 28, kThreads, 0, st>>>((const float*)workspace, nblk, C,

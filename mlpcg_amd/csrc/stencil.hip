@@ -544,8 +544,9 @@ static inline const int* done_for(const int* done, long voxels) {
   return voxels >= kDoneCheckMinVoxels ? done : nullptr;
 }
 
-// grids up to this size compute their stencils with one thread per (tap, voxel)
-constexpr long kSmallWeightsVoxels = 1L << 18;
+// grids up to this size (32^3) compute their stencils with one thread per (tap, voxel);
+// at 64^3 its 27 gathers per thread cost more than the per-voxel kernel (54 vs 20 us)
+constexpr long kSmallWeightsVoxels = 1L << 15;
 
 static inline dim3 grid1(long items, int y = 1) {
   return dim3((unsigned)((items + kThreads - 1) / kThreads), (unsigned)y, 1);

@@ -175,6 +175,48 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ p, uns
   return t;
 }
 
+// sum_partials for NR rows (row r at p + r * nb) with all rows' loads issued in one round;
+// per row the same accumulators and order as sum_partials, so the same results.  Valid
+// in thread 0.
+template <int NR>
+__device__ __forceinline__ void sum_partials_rows(const double* __restrict__ p, unsigned nb, int nr,
+                                                  double (&t)[NR]) {
+  __shared__ double sh[NR][kFinThreads / 64];
+  double a[NR][8];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[r][u] = 0.0;
+  unsigned b = threadIdx.x;
+  for (; b + 7 * kFinThreads < nb; b += 8 * kFinThreads) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r < nr)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[r][u] += p[(size_t)r * nb + b + u * kFinThreads];
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+    if (r < nr)
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (b + u * kFinThreads < nb) a[r][u] += p[(size_t)r * nb + b + u * kFinThreads];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const double s = wave_sum(((a[r][0] + a[r][1]) + (a[r][2] + a[r][3])) + ((a[r][4] + a[r][5]) + (a[r][6] + a[r][7])));
+    if ((threadIdx.x & 63) == 0) sh[r][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      double v = 0.0;
+      for (int w = 0; w < kFinThreads / 64; ++w) v += sh[r][w];
+      t[r] = v;
+    }
+  }
+}
+
 // Early exit flag shared by all kernels of a running solve: once the solve has
 // converged the queued kernels of a speculative iteration return immediately.
 __device__ __forceinline__ bool solve_done(const int* done) {

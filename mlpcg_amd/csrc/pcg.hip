@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
     }
   }
   double t[3] = {0.0, 0.0, 0.0};
-  for (int r = 0; r < nr; ++r) t[r] = sum_partials(partials + (size_t)r * nb, nb);
+  sum_partials_rows<3>(partials, nb, nr, t);
   if (threadIdx.x != 0) return;
   switch (op) {
     case kFinResidInit:

@@ -77,25 +77,6 @@ __device__ __forceinline__ TA upd(float x, float p, TA a, bool has_p) {
   return has_p ? (TA)(float)((TA)x + a * (TA)p) : (TA)x;
 }
 
-// Consecutive lanes of a wave own consecutive runs of a z-row (run index = wave base +
-// lane), so the z-halo of a run is the edge value of the neighbouring lane's centre row:
-// a DPP wave shift instead of a memory load.  wave_from_prev: lane l receives lane l-1's
-// value; wave_from_next: lane l+1's.  Lanes 0 / 63 receive garbage and load instead.
-__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ float wave_from_prev(float v) { return __builtin_bit_cast(float, dpp_prev(__builtin_bit_cast(int, v))); }
-__device__ __forceinline__ float wave_from_next(float v) { return __builtin_bit_cast(float, dpp_next(__builtin_bit_cast(int, v))); }
-__device__ __forceinline__ double wave_from_prev(double v) {
-  const long long u = __builtin_bit_cast(long long, v);
-  const long long r = ((long long)(unsigned)dpp_prev((int)(u >> 32)) << 32) | (unsigned)dpp_prev((int)u);
-  return __builtin_bit_cast(double, r);
-}
-__device__ __forceinline__ double wave_from_next(double v) {
-  const long long u = __builtin_bit_cast(long long, v);
-  const long long r = ((long long)(unsigned)dpp_next((int)(u >> 32)) << 32) | (unsigned)dpp_next((int)u);
-  return __builtin_bit_cast(double, r);
-}
-
 template <int VZ, typename TA>
 __device__ __forceinline__ void load_star(const float* __restrict__ x, const float* __restrict__ p,
                                           TA a, const Run<VZ>& r, int n1, int n2, int n3,

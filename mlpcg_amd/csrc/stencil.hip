@@ -426,16 +426,23 @@ __global__ __launch_bounds__(kThreads) void upsample2_kernel(UpArgs a) {
     // coarse z segment [k0/2 - 1, k0/2 + VZ/2] of the four rows, indices clamped to the
     // grid: with the clamp the interior weights reproduce the edge cases exactly
     // (f = 0 -> x[0]; f = 2m3-1 -> x[m3-1]), so the taps below are compile-time
+    // The two end cells come from the neighbouring lanes (consecutive lanes hold
+    // consecutive segments of a row) by DPP shifts; lanes 0 / 63 load them, and at the
+    // row ends the clamp makes them the segment's own end values.
     constexpr int NS = VZ / 2 + 2;
     const int kc = c.k0 >> 1;
     const int klo = max(kc - 1, 0), khi = min(kc + VZ / 2, m3 - 1);
+    const int lane = threadIdx.x & 63;
+    const bool row_start = c.k0 == 0, row_end = c.k0 + VZ >= a.n3;
     A seg[4][NS];
     const TI* rows[4] = {r00, r01, r10, r11};
 #pragma unroll
+    for (int q = 0; q < 4; ++q) load_vec<VZ / 2>(rows[q] + kc, &seg[q][1]);
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
-      load_vec<VZ / 2>(rows[q] + kc, &seg[q][1]);
-      seg[q][0] = (A)rows[q][klo];
-      seg[q][NS - 1] = (A)rows[q][khi];
+      const A lo = wave_from_prev(seg[q][VZ / 2]), hi = wave_from_next(seg[q][1]);
+      seg[q][0] = row_start ? seg[q][1] : (lane == 0 ? (A)rows[q][klo] : lo);
+      seg[q][NS - 1] = row_end ? seg[q][VZ / 2] : (lane == 63 ? (A)rows[q][khi] : hi);
     }
 #pragma unroll
     for (int t = 0; t < VZ; ++t) {

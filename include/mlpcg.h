@@ -178,10 +178,11 @@ void mlpcg_precond_destroy(mlpcg_precond* P);
  * with stored K); 2: on every such grid.  Process-wide; returns the previous mode. */
 int mlpcg_set_fused_apply(int mode);
 /* No counterpart in the reference: SmallSM plans (swap off, FP32/FP16/BF16) created after
- * this call run their two coarsest levels (at most 4096 voxels on level n-1) as one
- * workgroup kernel with the level vectors in LDS (enable = 1) instead of seven small
- * kernels (enable = 0, the default: at 256^3, l=5 the single workgroup takes 85 us
- * against 45 us for the seven kernels).  Process-wide; returns the previous setting. */
+ * this call run their two coarsest levels as one workgroup kernel with the level vectors
+ * in LDS (enable = 1, the default) instead of seven small kernels (enable = 0).  Used when
+ * level n-1 has even dims and at most 4096 voxels (16-bit storage, n3 % 4 == 0) or 2048
+ * (otherwise); at 256^3, l=5 in BF16 it takes 21 us against ~40 us of kernel time for
+ * the seven kernels.  Process-wide; returns the previous setting. */
 int mlpcg_set_coarse_tail(int enable);
 /* No counterpart in the reference (it has no plan object): y = block(x * (*in_scale)) for
  * stencil block `block` of the plan alone (block order of the weights array above; not a

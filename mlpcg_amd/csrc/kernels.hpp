@@ -68,8 +68,11 @@ int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream
 
 // The two coarsest levels of the SmallSM V-cycle in one workgroup (tail.hip): from the
 // output x_in of pre[n-2] (level n-2 grid) to up(c0 x + c1 post[n-1](up(l(pool(x))))) on
-// the level n-2 grid.  f1..f3: level n-1 dims (at most kTailMaxFine voxels, even).
+// the level n-2 grid.  f1..f3: level n-1 dims (even; at most kTailMaxFine voxels, and the
+// level n-1 and n grids with a one-cell halo at most kTailMaxHalo / kTailMaxHaloCoarse).
 constexpr int kTailMaxFine = 4096;
+constexpr int kTailMaxHalo = 8192;
+constexpr int kTailMaxHaloCoarse = 2048;
 struct TailArgs {
   const void* x_in;
   const void* K_pre;

@@ -30,8 +30,9 @@ def set_fused_apply(mode) -> int:
 
 
 def set_coarse_tail(enable) -> bool:
-    """Process-wide switch (default off) for SmallSM plans created afterwards: the two
-    coarsest levels run as one workgroup kernel when level n-1 has at most 4096 voxels.
+    """Process-wide switch (default on) for SmallSM plans created afterwards: the two
+    coarsest levels run as one workgroup kernel when level n-1 is small enough (even
+    dims; at most 4096 voxels with 16-bit storage and n3 % 4 == 0, else 2048).
     Returns the previous setting."""
     return bool(lib.mlpcg_set_coarse_tail(int(bool(enable))))
 

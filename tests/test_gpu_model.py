@@ -135,11 +135,11 @@ def test_plan_matches_module_eval_forward_f32(dev):
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float32, 2e-6), (torch.bfloat16, 2e-3), (torch.float16, 5e-4)])
 @pytest.mark.parametrize("n,shape,mode", [(3, (32, 32, 32), "trilinear"), (2, (16, 32, 16), "nearest"),
-                                          (4, (64, 64, 64), "trilinear")])
+                                          (4, (64, 64, 64), "trilinear"), (2, (16, 16, 12), "trilinear")])
 def test_coarse_tail_matches_kernel_sequence(dev, dtype, tol, n, shape, mode):
     """The one-workgroup kernel for the two coarsest levels computes the same V-cycle tail
     as the seven per-level kernels (same roundings to the storage type; FP32 summation
-    order may differ)."""
+    order may differ).  Level n-1 of (16, 16, 12) has n3 = 6: runs of 2 z cells."""
     from mlpcg_amd.precond import set_coarse_tail
     m, _ = _small_model(n, 1, mode, seed=31, dtype=torch.float32)
     img, r = _image(1, shape, 32).float().to(dev), _rand((1, 1, *shape), 33).float().to(dev)

@@ -608,6 +608,7 @@ static unsigned red_blocks(long items) {
 }
 
 size_t reduction_workspace_bytes() { return (size_t)kMaxRedBlocks * 4 * sizeof(double) + 256; }
+long reduction_capacity() { return (long)kMaxRedBlocks * 4; }
 
 int launch_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3, hipStream_t st) {
   const long V = (long)n1 * n2 * n3;

@@ -59,6 +59,8 @@ struct CsrArgs {
 };
 
 size_t reduction_workspace_bytes();
+// doubles of partial rows a reduction workspace holds (rows x blocks must not exceed it)
+long reduction_capacity();
 int launch_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3, hipStream_t st);
 int launch_resid_flat(const FlatArgs& a, int init, hipStream_t st);
 int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st);

@@ -726,10 +726,18 @@ int launch_avgpool2(int dtype, const void* in, void* out, int B, int n1, int n2,
   return check_launch("avgpool2");
 }
 
+// 16-bit outputs run 16 cells per thread (16-byte coarse loads, two 16-byte stores)
 template <typename TI, typename TO>
 static void launch_up_t(const UpArgs& a, hipStream_t st) {
-  const int vz = pick_vz_small(a.n3, 8, (long)a.n1 * a.n2 * a.n3 * a.B);
+  constexpr bool kNarrow = sizeof(TO) == 2 && sizeof(TI) == 2;
+  const int vz = pick_vz_small(a.n3, kNarrow ? 16 : 8, (long)a.n1 * a.n2 * a.n3 * a.B);
   const long items = (long)a.n1 * a.n2 * (a.n3 / vz);
+  if constexpr (kNarrow) {
+    if (vz == 16) {
+      upsample2_kernel<TI, TO, TO, 16><<<grid1(items, a.B), kThreads, 0, st>>>(a);
+      return;
+    }
+  }
   if (vz == 8) upsample2_kernel<TI, TO, TO, 8><<<grid1(items, a.B), kThreads, 0, st>>>(a);
   else if (vz == 4) upsample2_kernel<TI, TO, TO, 4><<<grid1(items, a.B), kThreads, 0, st>>>(a);
   else if (vz == 2) upsample2_kernel<TI, TO, TO, 2><<<grid1(items, a.B), kThreads, 0, st>>>(a);

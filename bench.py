@@ -79,6 +79,8 @@ def parse():
                     choices=("pillars256", "spheres128", "train256", "sphere64cpu"),
                     help="pillars256: the benchmark line; the others print their own report")
     ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--skip-heldout", action="store_true",
+                    help="train256: skip the held-out solves (kernel traces of the training steps alone)")
     ap.add_argument("--graphs", type=int, default=0, help="replay solver iterations as HIP graphs")
     return ap.parse_args()
 
@@ -310,7 +312,7 @@ def workload_train256(args, dev):
         its = []
         model.eval()
         with torch.no_grad():
-            for sb in held:
+            for sb in ([] if args.skip_heldout else held):
                 plan = model.make_preconditioner(sb.image, torch.float32)
                 s = FusedPCG(sb.scene.flags, "flags", device=dev)
                 _, info = s.solve(s.vector(S.make_rhs(sb.scene.flags, 0)[0]), plan, method=args.method,

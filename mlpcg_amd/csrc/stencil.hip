@@ -654,8 +654,11 @@ static void launch_apply_tr_t(const ApplyArgs& a, int vz, hipStream_t st) {
 
 int launch_sm_apply_transposed(int tk, const ApplyArgs& args, hipStream_t st) {
   MLPCG_REQUIRE(args.B >= 1 && args.n1 >= 1 && args.n2 >= 1 && args.n3 >= 1, "sm_apply_t: bad sizes");
-  const int vz = pick_vz_small(args.n3, tk == kF64 ? 2 : (tk == kF32 ? 4 : 8),
-                               (long)args.n1 * args.n2 * args.n3 * args.B);
+  // runs of 4 cells for every storage but FP64: with 8 16-bit cells the 27 K rows and their
+  // halos hold 256 VGPRs and the BF16 level-0 gather at 256^3 took 685 us against 534 us
+  // (halo cells from neighbouring lanes by DPP shifts instead of 2-byte loads: 639 us with
+  // 8 cells, 632 us with 4)
+  const int vz = pick_vz_small(args.n3, tk == kF64 ? 2 : 4, (long)args.n1 * args.n2 * args.n3 * args.B);
   switch (tk) {
     case kF64: launch_apply_tr_t<double>(args, vz, st); break;
     case kF32: launch_apply_tr_t<float>(args, vz, st); break;

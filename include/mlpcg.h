@@ -170,20 +170,26 @@ int mlpcg_precond_apply(mlpcg_precond* P, const void* r, void* z, const double* 
                         const uint8_t* out_mask, void* stream);
 size_t mlpcg_precond_bytes(const mlpcg_precond* P);
 void mlpcg_precond_destroy(mlpcg_precond* P);
-/* No counterpart in the reference (its SM blocks always evaluate the weighted stencil on
- * the fly): for BF16 plans created after this call, blocks on grids with
- * n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 2, not transposed) recompute K on the matrix
- * cores inside the apply instead of storing it.  mode 0: never (K stored like every other
- * block); 1 (the default): on such grids of at least 2^18 voxels (smaller ones are faster
- * with stored K); 2: on every such grid.  Process-wide; returns the previous mode. */
-int mlpcg_set_fused_apply(int mode);
-/* No counterpart in the reference: SmallSM plans (swap off, FP32/FP16/BF16) created after
- * this call run their two coarsest levels as one workgroup kernel with the level vectors
- * in LDS (enable = 1, the default) instead of seven small kernels (enable = 0).  Used when
- * level n-1 has even dims and at most 4096 voxels (16-bit storage, n3 % 4 == 0) or 2048
- * (otherwise); at 256^3, l=5 in BF16 it takes 21 us against ~40 us of kernel time for
- * the seven kernels.  Process-wide; returns the previous setting. */
-int mlpcg_set_coarse_tail(int enable);
+/* mlpcg_precond_create with the plan's execution policies given explicitly (create uses
+ * fused_mode 1 and coarse_tail 1).  No counterpart in the reference, whose SM blocks
+ * always evaluate the weighted stencil on the fly:
+ *   fused_mode: BF16 blocks on grids with n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 2, not
+ *     transposed) recompute K on the matrix cores inside the apply instead of storing it.
+ *     0: never (K stored like every other block); 1: on such grids of at least 2^18
+ *     voxels (smaller ones are faster with stored K); 2: on every such grid.
+ *   coarse_tail: SmallSM plans (swap off, FP32/FP16/BF16) run their two coarsest levels as
+ *     one workgroup kernel with the level vectors in LDS (1) instead of seven small kernels
+ *     (0), when level n-1 has even dims and at most 4096 voxels (16-bit storage,
+ *     n3 % 4 == 0) or 2048 (otherwise).
+ * The policies belong to the plan: plans with different policies coexist. */
+int mlpcg_precond_create_ex(mlpcg_precond** out, int kind, int n_levels, int C, int n1, int n2,
+                            int n3, int dtype, int nearest, int mask, int swap, int fused_mode,
+                            int coarse_tail, const void* image, const void* const* weights,
+                            const void* const* biases, void* stream);
+/* Number of blocks of the plan that use the fused matrix-core apply; whether the plan
+ * runs the coarse-tail kernel (1 / 0).  -1 for a null plan.  No reference counterpart. */
+int mlpcg_precond_fused_blocks(const mlpcg_precond* P);
+int mlpcg_precond_coarse_tail(const mlpcg_precond* P);
 /* No counterpart in the reference (it has no plan object): y = block(x * (*in_scale)) for
  * stencil block `block` of the plan alone (block order of the weights array above; not a
  * transposed one), on the grid of its level, without level combine or masks; x and y of

@@ -74,8 +74,10 @@ SIGNATURES = {
     "mlpcg_precond_apply": (i32, [vp, vp, vp, vp, vp, vp]),
     "mlpcg_precond_bytes": (sz, [vp]),
     "mlpcg_precond_destroy": (None, [vp]),
-    "mlpcg_set_fused_apply": (i32, [i32]),
-    "mlpcg_set_coarse_tail": (i32, [i32]),
+    "mlpcg_precond_create_ex": (i32, [C.POINTER(vp), i32, i32, i32, i32, i32, i32, i32, i32, i32, i32,
+                                      i32, i32, vp, C.POINTER(vp), C.POINTER(vp), vp]),
+    "mlpcg_precond_fused_blocks": (i32, [vp]),
+    "mlpcg_precond_coarse_tail": (i32, [vp]),
     "mlpcg_precond_apply_block": (i32, [vp, i32, vp, i32, vp, i32, vp, vp]),
     "mlpcg_solver_create_flags": (i32, [C.POINTER(vp), vp, i32, i32, i32, vp]),
     "mlpcg_solver_create_csr": (i32, [C.POINTER(vp), vp, vp, vp, i64, vp, i32, i32, i32, vp]),

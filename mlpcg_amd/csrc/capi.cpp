@@ -55,7 +55,7 @@ static int check_block_args(const char* who, int C, int B, int n1, int n2, int n
 extern "C" {
 
 const char* mlpcg_last_error(void) { return g_err; }
-int mlpcg_version(void) { return 100; }
+int mlpcg_version(void) { return 200; }
 
 // --------------------------------------------------------------------------- SM block
 size_t mlpcg_sm_block_workspace_size(int C, int B, int n1, int n2, int n3, int dtype) {
@@ -273,28 +273,43 @@ int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const voi
                              STREAM(stream));
 }
 
-size_t mlpcg_dot_workspace_size(void) { return reduction_workspace_bytes() + 256; }
+size_t mlpcg_dot_workspace_size(void) { return reduction_workspace_bytes(); }
 
 int mlpcg_dot(const float* x, const float* y, long n, double* out, void* workspace, void* stream) {
   MLPCG_REQUIRE(x && y && out && workspace, "dot: null pointer");
-  // workspace: [counter (256 B, zero on first use; re-armed by the kernel)] [partials]
-  unsigned* counter = (unsigned*)workspace;
-  double* partials = (double*)((char*)workspace + 256);
-  return launch_dot(x, y, n, partials, counter, out, STREAM(stream));
+  // workspace: the per-block partial sums of the two-kernel reduction
+  return launch_dot(x, y, n, (double*)workspace, out, STREAM(stream));
 }
 
 // --------------------------------------------------------------------------- plan
 int mlpcg_precond_create(mlpcg_precond** out, int kind, int n_levels, int C, int n1, int n2, int n3,
                          int dtype, int nearest, int mask, int swap, const void* image,
                          const void* const* weights, const void* const* biases, void* stream) {
+  return mlpcg_precond_create_ex(out, kind, n_levels, C, n1, n2, n3, dtype, nearest, mask, swap,
+                                 kDefaultFusedMode, kDefaultCoarseTail, image, weights, biases, stream);
+}
+
+int mlpcg_precond_create_ex(mlpcg_precond** out, int kind, int n_levels, int C, int n1, int n2, int n3,
+                            int dtype, int nearest, int mask, int swap, int fused_mode, int coarse_tail,
+                            const void* image, const void* const* weights, const void* const* biases,
+                            void* stream) {
   MLPCG_REQUIRE(out && image && weights && biases, "precond_create: null pointer");
   Precond* P = nullptr;
-  if (precond_create(&P, kind, n_levels, C, n1, n2, n3, dtype, nearest, mask, swap, image, weights,
-                     biases, STREAM(stream)))
+  if (precond_create(&P, kind, n_levels, C, n1, n2, n3, dtype, nearest, mask, swap, fused_mode,
+                     coarse_tail != 0, image, weights, biases, STREAM(stream)))
     return -1;
   *out = new mlpcg_precond{P};
   return 0;
 }
+
+int mlpcg_precond_fused_blocks(const mlpcg_precond* P) {
+  if (!P || !P->impl) return -1;
+  int nf = 0;
+  for (const Precond::Blk& b : P->impl->blocks) nf += b.fused;
+  return nf;
+}
+
+int mlpcg_precond_coarse_tail(const mlpcg_precond* P) { return (P && P->impl) ? P->impl->tail : -1; }
 
 int mlpcg_precond_update(mlpcg_precond* P, const void* image, const void* const* weights,
                          const void* const* biases, void* stream) {
@@ -320,18 +335,6 @@ int mlpcg_precond_apply_block(mlpcg_precond* P, int block, const void* x, int x_
                               int y_dtype, const double* in_scale, void* stream) {
   MLPCG_REQUIRE(P && P->impl && x && y, "precond_apply_block: null pointer");
   return P->impl->apply_block(block, x, x_dtype, y, y_dtype, in_scale, STREAM(stream));
-}
-
-int mlpcg_set_coarse_tail(int enable) {
-  const int prev = coarse_tail_enabled();
-  set_coarse_tail(enable);
-  return prev;
-}
-
-int mlpcg_set_fused_apply(int mode) {
-  const int prev = fused_apply_enabled();
-  set_fused_apply(mode);
-  return prev;
 }
 
 // --------------------------------------------------------------------------- solver

@@ -81,6 +81,7 @@ struct TailArgs {
   const float* c0;
   const float* c1;
   void* out;
+  const int* done;  // the solver's stop flag (every workgroup returns at once when set), or null
   int nearest;
   int f1, f2, f3;
 };

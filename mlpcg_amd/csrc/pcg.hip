@@ -186,8 +186,7 @@ __global__ __launch_bounds__(kThreads) void p_spmv_flat_kernel(FlatArgs a) {
 __global__ __launch_bounds__(kThreads) void rz_dot_kernel(const float* __restrict__ r,
                                                           const float* __restrict__ z,
                                                           const float* __restrict__ r_old, long n,
-                                                          SolveState* st, double* partials,
-                                                          unsigned* counter) {
+                                                          SolveState* st, double* partials) {
   if (st->done) return;
   double red[2] = {0.0, 0.0};
   // each thread owns runs of kDotRun consecutive floats, all loads issued before use so
@@ -329,8 +328,7 @@ __global__ __launch_bounds__(kThreads) void spmv_flat_kernel(const uint8_t* __re
 // sum_i a[i] b[i] in FP64 (drop-in dot for torch-free callers).
 __global__ __launch_bounds__(kThreads) void dot_kernel(const float* __restrict__ x,
                                                        const float* __restrict__ y, long n,
-                                                       double* partials, unsigned* counter,
-                                                       double* out) {
+                                                       double* partials) {
   double red[1] = {0.0};
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
     red[0] += (double)x[i] * (double)y[i];
@@ -652,9 +650,9 @@ int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, const float* aN, do
 }
 
 int launch_rz_dot(const float* r, const float* z, const float* r_old, long n, SolveState* s,
-                  double* partials, unsigned* counter, hipStream_t st) {
+                  double* partials, hipStream_t st) {
   const unsigned nb = red_blocks(n / kDotRun + 1);
-  rz_dot_kernel<<<nb, kThreads, 0, st>>>(r, z, r_old, n, s, partials, counter);
+  rz_dot_kernel<<<nb, kThreads, 0, st>>>(r, z, r_old, n, s, partials);
   finalize(kFinRZ, 2, partials, nb, s, nullptr, st);
   return check_launch("rz_dot");
 }
@@ -736,10 +734,10 @@ int launch_spmv_flat(const uint8_t* code, const float* x, float* y, int n1, int 
   return check_launch("spmv_flat");
 }
 
-int launch_dot(const float* x, const float* y, long n, double* partials, unsigned* counter,
+int launch_dot(const float* x, const float* y, long n, double* partials,
                double* out, hipStream_t st) {
   const unsigned nb = red_blocks(n);
-  dot_kernel<<<nb, kThreads, 0, st>>>(x, y, n, partials, counter, out);
+  dot_kernel<<<nb, kThreads, 0, st>>>(x, y, n, partials);
   finalize(kFinDot, 1, partials, nb, nullptr, out, st);
   return check_launch("dot");
 }

@@ -35,7 +35,6 @@ struct FlatArgs {
   float* w;
   SolveState* st;
   double* partials;
-  unsigned* counter;
   int n1, n2, n3;
 };
 
@@ -54,7 +53,6 @@ struct CsrArgs {
   float* w;
   SolveState* st;
   double* partials;
-  unsigned* counter;
   long n;
 };
 
@@ -67,7 +65,7 @@ int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st);
 int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, const float* aN, double* alfa_out,
                          hipStream_t st);
 int launch_rz_dot(const float* r, const float* z, const float* r_old, long n, SolveState* s,
-                  double* partials, unsigned* counter, hipStream_t st);
+                  double* partials, hipStream_t st);
 int launch_scatter(const int* idx, const float* src, float* grid, long n, const int* done,
                    hipStream_t st);
 int launch_gather(const int* idx, const float* grid, float* dst, long n, const int* done,
@@ -89,7 +87,7 @@ int launch_spmv_csr_f64(const int* rowptr, const int* col, const double* val, co
                         double* y, long n, hipStream_t st);
 int launch_spmv_flat(const uint8_t* code, const float* x, float* y, int n1, int n2, int n3,
                      hipStream_t st);
-int launch_dot(const float* x, const float* y, long n, double* partials, unsigned* counter,
+int launch_dot(const float* x, const float* y, long n, double* partials,
                double* out, hipStream_t st);
 
 }  // namespace mlpcg

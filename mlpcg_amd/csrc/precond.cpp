@@ -9,6 +9,7 @@
 //   kSmallSPD (SmallSPDSMModelDn3D, :388-444):
 //             l0[0..n-1], l                           then c0[0..n-1], c1[0..n-1]
 //             (K also holds l0_t[0..n-1], l_t)
+#include <atomic>
 #include <cstring>
 
 #include "runtime.hpp"
@@ -23,18 +24,16 @@ static size_t dsize(int dt) {
   }
 }
 
-// 0 off, 1 grids of at least kFusedMinVoxels (below that even 2-plane tiles are too few
-// to fill the GPU; at 64^3 with 2-plane tiles the fused apply already beats the stored
-// stencils: a 256^3 solve runs 0.75 ms faster with level 2 fused), 2 every eligible grid
-static int g_fused_apply = 1;
+// Fused mode 1 uses the matrix-core apply on grids of at least kFusedMinVoxels voxels:
+// below that even 2-plane tiles are too few to fill the GPU; at 64^3 with 2-plane tiles
+// the fused apply already beats the stored stencils (a 256^3 solve runs 0.75 ms faster
+// with level 2 fused).  The coarse tail is on by default: at 256^3, l=5 in BF16 the tail
+// kernel takes 21 us against ~40 us of kernel time (plus launch gaps) for the seven
+// per-level kernels it replaces.
 constexpr long kFusedMinVoxels = 1L << 18;
-void set_fused_apply(int mode) { g_fused_apply = (mode < 0 || mode > 2) ? 1 : mode; }
-int fused_apply_enabled() { return g_fused_apply; }
-// on by default: at 256^3, l=5 in BF16 the tail kernel takes 21 us against ~40 us of
-// kernel time (plus launch gaps) for the seven per-level kernels it replaces
-static int g_coarse_tail = 1;
-void set_coarse_tail(int enable) { g_coarse_tail = enable ? 1 : 0; }
-int coarse_tail_enabled() { return g_coarse_tail; }
+
+// plan generation ids: never reused within a process, unlike plan addresses
+static std::atomic<unsigned long> g_next_gen{1};
 
 Precond::~Precond() {
   for (void* p : allocs) (void)hipFree(p);
@@ -49,8 +48,9 @@ static int palloc(Precond* P, void** out, size_t bytes) {
 }
 
 int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3, int dtype,
-                   int nearest, int mask, int swap, const void* image, const void* const* weights,
-                   const void* const* biases, hipStream_t st) {
+                   int nearest, int mask, int swap, int fused_mode, int coarse_tail,
+                   const void* image, const void* const* weights, const void* const* biases,
+                   hipStream_t st) {
   MLPCG_REQUIRE(out != nullptr, "precond_create: null output handle");
   MLPCG_REQUIRE(kind >= kSmallSM && kind <= kSmallSPD, "precond_create: bad model kind %d", kind);
   MLPCG_REQUIRE(n >= 1 && n < kMaxLevels, "precond_create: levels must be in [1,%d)", kMaxLevels);
@@ -61,7 +61,9 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
                 "precond_create: grid (%d,%d,%d) must be divisible by 2^levels = %d", n1, n2, n3, div);
   MLPCG_REQUIRE(!mask || C >= 2, "precond_create: mask needs the fluid image channel (C >= 2)");
   MLPCG_REQUIRE(kind == kSmallSM || (!mask && !swap), "precond_create: mask/swap only for SmallSM");
+  MLPCG_REQUIRE(fused_mode >= 0 && fused_mode <= 2, "precond_create: fused mode must be 0, 1 or 2");
   Precond* P = new Precond();
+  P->gen = g_next_gen.fetch_add(1);
   P->kind = kind;
   P->n = n;
   P->C = C;
@@ -122,8 +124,8 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
     const int* d = P->dims[b.level];
     void* K = nullptr;
     void* wp = nullptr;
-    const bool big = P->V[b.level] >= kFusedMinVoxels || g_fused_apply == 2;
-    b.fused = (half && !b.trans && g_fused_apply && big && fused_eligible(C, 1, d[0], d[1], d[2])) ? 1 : 0;
+    const bool big = P->V[b.level] >= kFusedMinVoxels || fused_mode == 2;
+    b.fused = (half && !b.trans && fused_mode && big && fused_eligible(C, 1, d[0], d[1], d[2])) ? 1 : 0;
     if (b.fused) {
       TRY(palloc(P, &wp, fused_weights_bytes(C)));
       if (!P->imgw[b.level]) TRY(palloc(P, &P->imgw[b.level], (size_t)C * P->V[b.level] * es));
@@ -134,7 +136,7 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
     P->wpack.push_back(wp);
   }
   // the two coarsest SmallSM levels in one workgroup kernel when they are small enough
-  P->tail = (g_coarse_tail && kind == kSmallSM && !swap && n >= 2 &&
+  P->tail = (coarse_tail && kind == kSmallSM && !swap && n >= 2 &&
              tail_eligible(dtype, P->dims[n - 1][0], P->dims[n - 1][1], P->dims[n - 1][2]) &&
              !P->blocks[n - 1].fused && !P->blocks[2 * n - 1].fused && !P->blocks[2 * n].fused)
                 ? 1
@@ -307,6 +309,7 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
     t.c0 = (const float*)c0p(n - 1, 0);
     t.c1 = (const float*)c1p(n - 1, 0);
     t.out = tmp[n - 2];
+    t.done = done;
     t.nearest = nearest;
     t.f1 = dims[n - 1][0];
     t.f2 = dims[n - 1][1];

@@ -20,6 +20,7 @@ enum ModelKind : int { kSmallSM = 0, kSPD = 1, kSmallSPD = 2 };
 // This is the caching that SmallSMModelDn3D.eval_forward does for imgs/c0/c1 through
 // its list arguments (sm_model_3d.py:180-195), extended to K.
 struct Precond {
+  unsigned long gen = 0;            // unique per plan of the process (graph caches key on it)
   int kind = kSmallSM;
   int n = 0, C = 1, dtype = kF32, acc = kF32;
   int nearest = 0, mask = 0, swap = 0;
@@ -65,20 +66,17 @@ struct Precond {
   ~Precond();
 };
 
-// Plans created while this is set use the fused matrix-core apply for eligible BF16
-// blocks: 0 never, 1 (default) on grids of at least 2^20 voxels, 2 on every eligible grid
-// (process-wide).
-void set_fused_apply(int mode);
-int fused_apply_enabled();
-
-// Plans created while this is set run the two coarsest SmallSM levels in one workgroup
-// kernel when they fit (default off).
-void set_coarse_tail(int enable);
-int coarse_tail_enabled();
+// Per-plan policies.  fused_mode: eligible BF16 blocks recompute their stencils on the
+// matrix cores inside the apply -- 0 never, 1 (default) on grids of at least
+// kFusedMinVoxels voxels, 2 on every eligible grid.  coarse_tail: the two coarsest SmallSM
+// levels run as one workgroup kernel when they fit (default on).
+constexpr int kDefaultFusedMode = 1;
+constexpr int kDefaultCoarseTail = 1;
 
 int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3, int dtype,
-                   int nearest, int mask, int swap, const void* image, const void* const* weights,
-                   const void* const* biases, hipStream_t st);
+                   int nearest, int mask, int swap, int fused_mode, int coarse_tail,
+                   const void* image, const void* const* weights, const void* const* biases,
+                   hipStream_t st);
 
 struct SolveInfo {
   int iters;
@@ -108,7 +106,6 @@ struct Solver {
   SolveState* st = nullptr;
   SolveState* st_host = nullptr;   // pinned ring of status snapshots
   double* partials = nullptr;
-  unsigned* counter = nullptr;
   double* history = nullptr;
   int hist_cap = 0;
   hipStream_t side = nullptr;
@@ -120,7 +117,7 @@ struct Solver {
   hipStream_t work = nullptr;
   hipEvent_t ev_in = nullptr;
   hipGraphExec_t graph_exec[2] = {};
-  const Precond* graph_P = nullptr;
+  unsigned long graph_P = 0;  // generation id of the plan the graphs were captured with
   const float* graph_b = nullptr;
   int graph_method = -1;
 

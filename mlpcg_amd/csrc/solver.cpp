@@ -54,7 +54,6 @@ static int solver_common(Solver* S, hipStream_t st) {
   if (salloc(S, (void**)&S->w, vb, true, st)) return -1;
   if (salloc(S, (void**)&S->st, sizeof(SolveState), true, st)) return -1;
   if (salloc(S, (void**)&S->partials, reduction_workspace_bytes(), true, st)) return -1;
-  if (salloc(S, (void**)&S->counter, 256, true, st)) return -1;
   MLPCG_HIP(hipHostMalloc((void**)&S->st_host, 4 * sizeof(SolveState), hipHostMallocDefault));
   memset(S->st_host, 0, 4 * sizeof(SolveState));
   MLPCG_HIP(hipStreamCreateWithFlags(&S->side, hipStreamNonBlocking));
@@ -132,7 +131,7 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
     MLPCG_HIP(hipEventRecord(ev_in, st));
     MLPCG_HIP(hipStreamWaitEvent(work, ev_in, 0));
     run = work;
-    if (graph_P != P || graph_b != b || graph_method != method) drop_graphs();
+    if (graph_P != (P ? P->gen : 0) || graph_b != b || graph_method != method) drop_graphs();
   }
   // initial state
   SolveState init;
@@ -159,7 +158,6 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
   fa0.b = b;
   fa0.st = this->st;
   fa0.partials = partials;
-  fa0.counter = counter;
   fa0.n1 = n1;
   fa0.n2 = n2;
   fa0.n3 = n3;
@@ -171,7 +169,6 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
   ca0.b = b;
   ca0.st = this->st;
   ca0.partials = partials;
-  ca0.counter = counter;
   ca0.n = n;
 
   int rc = 0;
@@ -229,7 +226,7 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
       if (nbd)
         TRY2(launch_finalize_rz(partials, nbd, this->st, s));
       else
-        TRY2(launch_rz_dot(rk, z, r[nxt], n, this->st, partials, counter, s));
+        TRY2(launch_rz_dot(rk, z, r[nxt], n, this->st, partials, s));
       if (layout == 0) {
         fa.z = z;
         fa.p_old = p[cur];
@@ -324,7 +321,7 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
         const hipError_t ie = hipGraphInstantiate(&graph_exec[par], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         MLPCG_HIP(ie);
-        graph_P = P;
+        graph_P = P ? P->gen : 0;
         graph_b = b;
         graph_method = method;
       }
@@ -352,6 +349,10 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
   SolveState fin;
   MLPCG_HIP(hipMemcpy(&fin, this->st, sizeof(SolveState), hipMemcpyDeviceToHost));
   const int iters = fin.iter;
+  // the count comes from device memory and sizes the copy into the caller's history
+  // (max_it + 1 doubles): refuse anything outside [0, max_it] instead of trusting it
+  MLPCG_REQUIRE(iters >= 0 && iters <= max_it, "solve: device iteration count %d outside [0, %d]", iters,
+                max_it);
   MLPCG_HIP(hipMemcpyAsync(x_out, x[iters & 1], vb, hipMemcpyDeviceToDevice, run));
   if (history_out)
     MLPCG_HIP(hipMemcpy(history_out, history, (size_t)(iters + 1) * sizeof(double), hipMemcpyDeviceToHost));
@@ -373,7 +374,7 @@ void Solver::drop_graphs() {
     if (graph_exec[i]) (void)hipGraphExecDestroy(graph_exec[i]);
     graph_exec[i] = nullptr;
   }
-  graph_P = nullptr;
+  graph_P = 0;
   graph_b = nullptr;
   graph_method = -1;
 }

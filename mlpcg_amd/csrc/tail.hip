@@ -204,6 +204,7 @@ __device__ __forceinline__ void tail_prefetch(const TailArgs& a, int helper) {
 
 template <typename T, int Z>
 __global__ __launch_bounds__(kTailThreads) void coarse_tail_kernel(TailArgs a) {
+  if (solve_done(a.done)) return;  // uniform over the grid: a queued iteration after convergence
   if (blockIdx.x != 0) {
     if (blockIdx.x % 8 == 0) tail_prefetch<T>(a, blockIdx.x / 8 - 1);
     return;

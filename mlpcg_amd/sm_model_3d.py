@@ -400,10 +400,11 @@ class SmallSMModelDn3D(BaseModel):
         lin = list(self.c0) + list(self.c1)
         return [m.weight for m in blocks + lin], [m.bias for m in blocks + lin]
 
-    def make_preconditioner(self, image, dtype=torch.float32):
-        """Native plan with K, the image pyramid and c0/c1 precomputed for `image`."""
+    def make_preconditioner(self, image, dtype=torch.float32, fused_mode=1, coarse_tail=True):
+        """Native plan with K, the image pyramid and c0/c1 precomputed for `image`;
+        fused_mode / coarse_tail are the plan's execution policies (precond.Preconditioner)."""
         from .precond import Preconditioner
-        return Preconditioner(self, image, dtype)
+        return Preconditioner(self, image, dtype, fused_mode=fused_mode, coarse_tail=coarse_tail)
 
 
 ####################################
@@ -473,9 +474,9 @@ class SPDSMModelDn3D(BaseModel):
         lin = list(self.c0) + list(self.c1)
         return [m.weight for m in blocks + lin], [m.bias for m in blocks + lin]
 
-    def make_preconditioner(self, image, dtype=torch.float32):
+    def make_preconditioner(self, image, dtype=torch.float32, fused_mode=1, coarse_tail=True):
         from .precond import Preconditioner
-        return Preconditioner(self, image, dtype)
+        return Preconditioner(self, image, dtype, fused_mode=fused_mode, coarse_tail=coarse_tail)
 
 
 ##########################################
@@ -531,9 +532,9 @@ class SmallSPDSMModelDn3D(BaseModel):
         lin = list(self.c0) + list(self.c1)
         return [m.weight for m in blocks + lin], [m.bias for m in blocks + lin]
 
-    def make_preconditioner(self, image, dtype=torch.float32):
+    def make_preconditioner(self, image, dtype=torch.float32, fused_mode=1, coarse_tail=True):
         from .precond import Preconditioner
-        return Preconditioner(self, image, dtype)
+        return Preconditioner(self, image, dtype, fused_mode=fused_mode, coarse_tail=coarse_tail)
 
 
 __all__ = [

@@ -18,6 +18,7 @@ import torch
 
 from . import _lib, kernels
 from ._lib import lib, check, ptr, stream_ptr
+from .precond import _require_vec
 from .scenes import FLUID
 from .timer import GlobalClock
 
@@ -58,7 +59,13 @@ class FusedPCG:
 
     def __init__(self, flags, layout="flags", A=None, device="cuda", graphs=False):
         self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise _lib.MlpcgError(f"FusedPCG runs on the GPU, not on {self.device}")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         flags_np = np.asarray(flags)
+        if flags_np.ndim != 3:
+            raise _lib.MlpcgError(f"flags must be a 3-d grid, got shape {flags_np.shape}")
         self.shape = tuple(int(s) for s in flags_np.shape)
         self.layout = layout
         self._flags = torch.as_tensor(flags_np.astype(np.int8), device=self.device).contiguous()
@@ -104,9 +111,25 @@ class FusedPCG:
     def solve(self, b, precond=None, x0=None, method="npcg", max_it=100, tol=1e-6, atol=1e-12,
               history=False, out=None):
         """-> (x, info dict).  b: float32 device vector in this layout."""
+        if method not in METHODS:
+            raise _lib.MlpcgError(f"unknown method {method}")
+        if int(max_it) < 1:
+            raise _lib.MlpcgError("max_it must be >= 1")
+        if not isinstance(b, torch.Tensor) or b.device != self.device:
+            raise _lib.MlpcgError(f"b must be a tensor on {self.device}")
         b = b.to(torch.float32).contiguous()
+        _require_vec(b, "b", torch.float32, self.n, self.device)
+        if x0 is not None:
+            _require_vec(x0, "x0", torch.float32, self.n, self.device)
         x = out if out is not None else torch.empty_like(b)
-        hist = np.zeros(max_it + 1, dtype=np.float64) if history else None
+        _require_vec(x, "out", torch.float32, self.n, self.device)
+        if precond is not None:
+            if precond.device != self.device or precond.shape != self.shape:
+                raise _lib.MlpcgError(f"preconditioner grid {precond.shape} on {precond.device} does "
+                                      f"not match the solver grid {self.shape} on {self.device}")
+            if precond.dtype == torch.float64:
+                raise _lib.MlpcgError("FP64 plans are for checking only; use float32/bfloat16/float16")
+        hist = np.zeros(int(max_it) + 1, dtype=np.float64) if history else None
         info = _lib.SolveInfo()
         P = precond.handle if precond is not None else None
         check(lib.mlpcg_solver_solve(self._h, P, ptr(b), ptr(x0), ptr(x), METHODS[method], int(max_it),

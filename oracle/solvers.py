@@ -12,14 +12,17 @@ from __future__ import annotations
 import numpy as np
 
 
-def npcg(b, A, x0, predict, max_it, tol=1e-10, atol=1e-12):
-    """cg_tests.py:26-101.  Returns (x, iterations, residual history)."""
+def npcg(b, A, x0, predict, max_it, tol=1e-10, atol=1e-12, trace=None):
+    """cg_tests.py:26-101.  Returns (x, iterations, residual history).  trace: optional
+    list that receives each search direction p (for property checks)."""
     norm_b = np.linalg.norm(b)
     r = b - A @ x0
     hist = [np.linalg.norm(r) / norm_b]
     x = x0.copy()
     z = predict(r)
     p = z
+    if trace is not None:
+        trace.append(p.copy())
     w = A @ p
     rz_old = r.dot(z)
     alpha = rz_old / p.dot(w)
@@ -33,6 +36,8 @@ def npcg(b, A, x0, predict, max_it, tol=1e-10, atol=1e-12):
         rz = r.dot(z)
         beta = rz / rz_old
         p = z + beta * p
+        if trace is not None:
+            trace.append(p.copy())
         w = A @ p
         alpha = rz / p.dot(w)
         x = x + alpha * p
@@ -44,14 +49,17 @@ def npcg(b, A, x0, predict, max_it, tol=1e-10, atol=1e-12):
     return x, k, hist
 
 
-def npcg_flex(b, A, x0, predict, max_it, tol=1e-10, atol=1e-12):
-    """cg_tests.py:103-183: beta = z.(r - r_old) / rz_old (Polak-Ribiere form)."""
+def npcg_flex(b, A, x0, predict, max_it, tol=1e-10, atol=1e-12, trace=None):
+    """cg_tests.py:103-183: beta = z.(r - r_old) / rz_old (Polak-Ribiere form).  trace:
+    optional list that receives each search direction p."""
     norm_b = np.linalg.norm(b)
     r = b - A @ x0
     hist = [np.linalg.norm(r) / norm_b]
     x = x0.copy()
     z = predict(r)
     p = z
+    if trace is not None:
+        trace.append(p.copy())
     w = A @ p
     rz_old = r.dot(z)
     r_old = r
@@ -66,6 +74,8 @@ def npcg_flex(b, A, x0, predict, max_it, tol=1e-10, atol=1e-12):
         rz = r.dot(z)
         beta = z.dot(r - r_old) / rz_old
         p = z + beta * p
+        if trace is not None:
+            trace.append(p.copy())
         w = A @ p
         alpha = rz / p.dot(w)
         x = x + alpha * p
@@ -78,9 +88,11 @@ def npcg_flex(b, A, x0, predict, max_it, tol=1e-10, atol=1e-12):
     return x, k, hist
 
 
-def npsd(b, A, x0, predict, max_it, tol=1e-10, atol=1e-10, num_prev=2):
+def npsd(b, A, x0, predict, max_it, tol=1e-10, atol=1e-10, num_prev=2, trace=None):
     """cg_tests.py:189-284: steepest descent A-orthogonalised against the last
-    num_prev directions; stop when ||r||/||b|| < max(tol, atol/||b||)."""
+    num_prev directions; stop when ||r||/||b|| < max(tol, atol/||b||).  trace: optional
+    list that receives, per iteration, (new direction, the kept directions it was
+    projected against, residual after the step)."""
     norm_b = np.linalg.norm(b)
     r = b - A @ x0
     hist = [np.linalg.norm(r) / norm_b]
@@ -89,6 +101,7 @@ def npsd(b, A, x0, predict, max_it, tol=1e-10, atol=1e-10, num_prev=2):
     norm = hist[0]
     for i in range(1, max_it + 1):
         q = predict(r)
+        kept = [v.copy() for v in p] if trace is not None else None
         if num_prev > 0:
             if i > num_prev:
                 for j in reversed(range(num_prev)):
@@ -115,6 +128,8 @@ def npsd(b, A, x0, predict, max_it, tol=1e-10, atol=1e-10, num_prev=2):
             beta = q.dot(r) / alpha
         x = x + q * beta
         r = b - A @ x
+        if trace is not None:
+            trace.append((q.copy(), kept, r.copy()))
         norm = np.linalg.norm(r) / norm_b
         hist.append(norm)
         if norm < max(tol, atol / norm_b):

@@ -133,3 +133,52 @@ def test_hip_spmv_matches_golden_matrix(dev):
     xg = torch.as_tensor(S.expand(x, flags).ravel(), device=dev)
     yf = K.spmv_flags(code, xg).cpu().numpy()[flags.ravel() == S.FLUID]
     assert np.array_equal(yf, ref)
+
+
+# --------------------------------------------- fixtures written by the reference's own model
+
+REF_CASES = ["c1_n2_8", "c1_n3_16", "c3_n2_16", "c3_n3_16"]
+
+
+@pytest.mark.parametrize("name", REF_CASES)
+def test_oracle_model_matches_reference_model(name):
+    """The oracle's composition (nearest upsample, no mask, no swap) reproduces
+    SmallSMModelDn3DPY (sm_model_3d_py.py:33-63) run by tests/golden/make_ref_model.py,
+    and its SMLinear coefficients equal the reference's Conv3d(C->27).mean() per level."""
+    z = _load("ref_models.npz")
+    n = int(z[f"{name}_n"])
+    P = _params(z, name)
+    img, b = _t(z[f"{name}_img"]), _t(z[f"{name}_b"])
+    y = M.model_forward(P, n, img, b, mode="nearest", mask=False, swap=False)
+    ref = z[f"{name}_y"]
+    assert np.abs(y.numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    im = img
+    for i in range(n):
+        for which in ("c0", "c1"):
+            c, _ = M.sm_linear(im, P[f"{which}.{i}.weight"], P[f"{which}.{i}.bias"])
+            assert abs(float(c) - float(z[f"{name}_{which}_{i}"])) < 1e-13
+        im = M.avg_pool(im)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", REF_CASES)
+def test_hip_model_and_plan_match_reference_model(dev, name):
+    """The drop-in module (batched forward) and the native plan (one vector at a time)
+    in FP64 reproduce the reference model's outputs."""
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    z = _load("ref_models.npz")
+    n = int(z[f"{name}_n"])
+    P = _params(z, name)
+    img, b, ref = _t(z[f"{name}_img"]).to(dev), _t(z[f"{name}_b"]).to(dev), z[f"{name}_y"]
+    C = int(img.shape[0])
+    m = SmallSMModelDn3D(n, C, "nearest", False, False).to(dev, torch.float64)
+    with torch.no_grad():
+        for k, p in m.named_parameters():
+            p.copy_(P[k])
+        y = m(img, b).cpu().numpy()
+    scale = np.abs(ref).max()
+    assert np.abs(y - ref).max() <= 1e-11 * scale
+    plan = m.make_preconditioner(img, torch.float64)
+    for bi in range(b.shape[0]):
+        zp = plan.apply(b[bi, 0].contiguous()).cpu().numpy()
+        assert np.abs(zp - ref[bi, 0]).max() <= 1e-11 * scale

@@ -18,12 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _plan(m, img, dtype, fused):
     """fused: 2 forces the fused kernel on every eligible grid (small test grids), 0 off."""
-    from mlpcg_amd.precond import set_fused_apply
-    prev = set_fused_apply(2 if fused else 0)
-    try:
-        return m.make_preconditioner(img, dtype)
-    finally:
-        set_fused_apply(prev)
+    return m.make_preconditioner(img, dtype, fused_mode=2 if fused else 0)
 
 
 def _exact_model(n, C, mask, seed):

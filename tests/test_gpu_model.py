@@ -140,16 +140,12 @@ def test_coarse_tail_matches_kernel_sequence(dev, dtype, tol, n, shape, mode):
     """The one-workgroup kernel for the two coarsest levels computes the same V-cycle tail
     as the seven per-level kernels (same roundings to the storage type; FP32 summation
     order may differ).  Level n-1 of (16, 16, 12) has n3 = 6: runs of 2 z cells."""
-    from mlpcg_amd.precond import set_coarse_tail
     m, _ = _small_model(n, 1, mode, seed=31, dtype=torch.float32)
     img, r = _image(1, shape, 32).float().to(dev), _rand((1, 1, *shape), 33).float().to(dev)
     z = {}
     for tail in (True, False):
-        prev = set_coarse_tail(tail)
-        try:
-            plan = m.make_preconditioner(img, dtype)
-        finally:
-            set_coarse_tail(prev)
+        plan = m.make_preconditioner(img, dtype, coarse_tail=tail)
+        assert plan.coarse_tail == tail
         z[tail] = plan.apply(r.reshape(shape)).double()
     assert float((z[True] - z[False]).norm() / z[False].norm()) < tol
 

@@ -73,3 +73,64 @@ def test_stopping_rule_counts():
     assert k == 5 and len(hist) == 6
     _, k, hist = O.npsd(b, A, np.zeros_like(b), lambda r: r.copy(), 5, tol=1e-30)
     assert k == 5 and len(hist) == 6
+
+
+def _nonsymmetric_p(A):
+    """A fixed, non-symmetric linear preconditioner: z = D^-1 (r + 0.3 L r) with L the
+    strictly lower part of A (the learned SM preconditioner is non-symmetric too)."""
+    import scipy.sparse as sp
+    d = A.diagonal()
+    L = sp.tril(A, k=-1).tocsr()
+    M = sp.diags(1.0 / d) @ (sp.identity(A.shape[0]) + 0.3 * L)
+    assert abs(M - M.T).max() > 1e-3
+    return lambda r: M @ r
+
+
+def _a_cos(A, u, v):
+    return abs(u @ (A @ v)) / np.sqrt((u @ (A @ u)) * (v @ (A @ v)))
+
+
+def test_npcg_flex_directions_are_a_conjugate_with_nonsymmetric_p():
+    """cg_tests.py:159: beta = z_k.(r_k - r_{k-1}) / rz_{k-1}.  With r_k = r_{k-1} -
+    alpha A p_{k-1} and alpha = rz_{k-1} / p_{k-1}.A p_{k-1}, this beta is exactly the one
+    that makes p_k A-conjugate to p_{k-1} for ANY preconditioner.  Fletcher-Reeves
+    (npcg, beta = rz_k / rz_{k-1}) is not, once P is non-symmetric: the check tells the
+    two apart."""
+    _, A, b = _problem()
+    P = _nonsymmetric_p(A)
+    dirs = []
+    O.npcg_flex(b, A, np.zeros_like(b), P, 25, tol=1e-30, trace=dirs)
+    assert len(dirs) == 25
+    flex = max(_a_cos(A, dirs[k], dirs[k - 1]) for k in range(1, len(dirs)))
+    assert flex < 1e-9, flex
+    fr_dirs = []
+    O.npcg(b, A, np.zeros_like(b), P, 25, tol=1e-30, trace=fr_dirs)
+    fr = max(_a_cos(A, fr_dirs[k], fr_dirs[k - 1]) for k in range(1, len(fr_dirs)))
+    assert fr > 1e-3, fr
+
+
+def test_npsd_directions_a_orthogonal_and_exact_line_search():
+    """cg_tests.py:217-281: each new direction Q is A-orthogonal to the (up to two) kept
+    directions it was projected against, and the step along Q is an exact line search,
+    so the new residual is orthogonal to Q.  In exact arithmetic both hold for any P:
+    the kept pair is itself A-orthogonal (by induction), so projecting against the newer
+    one first does not spoil the older one.  A sign error in a projection or in the step,
+    a projection against the wrong slot, or a dropped projection breaks one of them."""
+    _, A, b = _problem()
+    P = _nonsymmetric_p(A)
+    tr = []
+    O.npsd(b, A, np.zeros_like(b), P, 20, tol=1e-30, trace=tr)
+    assert len(tr) == 20
+    r_prev = b.copy()
+    for i, (Q, kept, r_new) in enumerate(tr):
+        assert len(kept) == min(i, 2)
+        for v in kept:
+            assert _a_cos(A, Q, v) < 1e-9, (i, _a_cos(A, Q, v))
+        assert abs(Q @ r_new) < 1e-10 * np.linalg.norm(Q) * np.linalg.norm(r_prev), i
+        r_prev = r_new
+    # without the projections (num_prev = 0: plain steepest descent) consecutive
+    # directions are far from A-orthogonal: the orthogonality check has teeth
+    tr0 = []
+    O.npsd(b, A, np.zeros_like(b), P, 20, tol=1e-30, num_prev=0, trace=tr0)
+    worst = max(_a_cos(A, tr0[i][0], tr0[i - 1][0]) for i in range(1, len(tr0)))
+    assert worst > 1e-2, worst

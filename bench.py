@@ -207,9 +207,22 @@ def train_rate(dev, steps=3, warmup=1, n=128, scenes=8, levels=5, graphed_steps=
             "last_loss": loss, "peak_hbm_GiB": round(torch.cuda.max_memory_allocated(dev) / 2 ** 30, 2)}
 
 
-def cpu_baseline(flags, image_np, model, iters, levels):
+def precond_probe(flags):
+    """The preconditioner input of the precision check: N(0,1) on the fluid cells (seed 0),
+    unit norm, on the full grid (float32, host)."""
+    from mlpcg_amd import scenes as S
+    fluid = flags.ravel() == S.FLUID
+    r = np.random.default_rng(0).standard_normal(int(fluid.sum()))
+    g = np.zeros(flags.size, dtype=np.float32)
+    g[fluid] = r / np.linalg.norm(r)
+    return g.reshape(flags.shape)
+
+
+def cpu_baseline(flags, image_np, model, iters, levels, plan_outputs=None):
     """The oracle (FP32 torch on the host cores) for one 256^3 preconditioner apply plus
-    the per-iteration SpMVs and dots with scipy, extrapolated to `iters` iterations."""
+    the per-iteration SpMVs and dots with scipy, extrapolated to `iters` iterations.  As
+    the checker, the same leg also runs the oracle in FP64 on the same input and returns
+    the relative error of each GPU plan's output (plan_outputs: name -> host tensor)."""
     import torch
     import scipy.sparse  # noqa: F401
     from mlpcg_amd import scenes as S
@@ -219,22 +232,27 @@ def cpu_baseline(flags, image_np, model, iters, levels):
     P = {k: v.detach().float().cpu() for k, v in model.state_dict().items()}
     img = torch.as_tensor(image_np, dtype=torch.float32)
     A = S.build_matrix(flags)
-    fluid = flags.ravel() == S.FLUID
-    rng = np.random.default_rng(0)
-    r = rng.standard_normal(A.shape[0])
-    g = np.zeros(flags.size, dtype=np.float32)
-    g[fluid] = r / np.linalg.norm(r)
+    g = precond_probe(flags)
+    r = g.ravel()[flags.ravel() == S.FLUID].astype(np.float64)
     x = torch.as_tensor(g).reshape(1, 1, *flags.shape)
     with torch.no_grad():
         t_apply, _ = _wall(lambda: M.model_forward(P, levels, img, x))
     t_spmv, _ = _wall(lambda: (A @ r, A @ r))
     t_dots, _ = _wall(lambda: (r @ r, r @ r, r @ r))
     per_it = t_apply + t_spmv + t_dots
-    return {"value": per_it * iters, "unit": "s", "cores": cores, "kind": "port",
-            "sample": (f"oracle FP32 torch preconditioner apply at 256^3 ({t_apply:.2f} s) + 2 scipy "
-                       f"SpMV + 3 dots ({t_spmv + t_dots:.3f} s) on {cores} host threads, "
-                       f"x {iters} iterations (extrapolated, FP32-preconditioner iteration count)"),
-            "per_iteration_s": per_it}
+    out = {"value": per_it * iters, "unit": "s", "cores": cores, "kind": "port",
+           "sample": (f"oracle FP32 torch preconditioner apply at 256^3 ({t_apply:.2f} s) + 2 scipy "
+                      f"SpMV + 3 dots ({t_spmv + t_dots:.3f} s) on {cores} host threads, "
+                      f"x {iters} iterations (extrapolated, FP32-preconditioner iteration count)"),
+           "per_iteration_s": per_it}
+    if plan_outputs:
+        P64 = {k: v.double() for k, v in P.items()}
+        with torch.no_grad():
+            z64 = M.model_forward(P64, levels, img.double(), x.double()).reshape(flags.shape)
+        nz = float(z64.norm())
+        out["oracle_fp64_rel_err"] = {k: float((z.double().reshape(flags.shape) - z64).norm() / nz)
+                                      for k, z in plan_outputs.items()}
+    return out
 
 
 def _wall(fn):
@@ -441,13 +459,16 @@ def main():
         x = rres.pop("x").cpu().numpy()[fluid].astype(np.float64)
         rres["rel_res_host_fp64"] = float(np.linalg.norm(b_np - A @ x) / nb)
 
-    # preconditioner error vs the FP64 plan on the same input
-    plan64 = model.make_preconditioner(image, torch.float64)
-    z64 = plan64.apply(r0.double() / r0.double().norm())
-    for name, p in plans.items():
-        z = p.apply(r0 / r0.norm()).double()
-        results[name]["precond_rel_err_vs_fp64"] = float((z - z64).norm() / z64.norm())
-    del plan64, z64
+    # every plan on the precision probe; the FP64 oracle output it is compared with comes
+    # from the CPU leg (the FP64 library plan stands in when that leg is skipped)
+    probe = torch.as_tensor(precond_probe(sc.flags), device=dev)
+    plan_out = {name: p.apply(probe).cpu() for name, p in plans.items()}
+    if args.skip_cpu:
+        plan64 = model.make_preconditioner(image, torch.float64)
+        z64 = plan64.apply(probe.double()).cpu()
+        for name, z in plan_out.items():
+            results[name]["precond_rel_err_vs_fp64_plan"] = float((z.double() - z64).norm() / z64.norm())
+        del plan64, z64
 
     # roofline of the dominant kernel: the fused level-0 block of the BF16 plan.  Its
     # arithmetic intensity (FUSED_FLOP_PER_VOXEL / 8 B = 196 FLOP/B) is below the
@@ -483,7 +504,10 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        cpu = cpu_baseline(sc.flags, image_np, model, results["fp32"]["iters"], args.levels)
+        cpu = cpu_baseline(sc.flags, image_np, model, results["fp32"]["iters"], args.levels, plan_out)
+        for name, e in cpu.pop("oracle_fp64_rel_err").items():
+            results[name]["precond_rel_err_vs_oracle_fp64"] = e
+            log(f"[bench] preconditioner {name}: relative error vs the FP64 oracle {e:.2e}")
         log(f"[bench] cpu baseline: {cpu['value']:.1f} s (extrapolated)")
 
     bf, f32 = results["bf16"], results["fp32"]

@@ -149,6 +149,21 @@ int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const voi
 size_t mlpcg_dot_workspace_size(void);
 int mlpcg_dot(const float* x, const float* y, long n, double* out, void* workspace, void* stream);
 
+/* ---- Lanczos vector kernels, FP64 (lib/create_dataset.py:28-52 _lanczos_algorithm) --
+ * The reference runs its Lanczos process in numpy; these are the device steps of the
+ * same recurrence (the SpMV is mlpcg_spmv_csr with MLPCG_F64, the reorthogonalisation a
+ * GEMV).  Scalars are device FP64 values. */
+size_t mlpcg_dot_f64_workspace_size(void);
+/* *out = x . y (FP64, deterministic two-kernel reduction; x, y 16-byte aligned) -- the
+ * w.dot(V[j]) and norm of create_dataset.py:38,42,48 */
+int mlpcg_dot_f64(const double* x, const double* y, long n, double* out, void* workspace, void* stream);
+/* w -= (*alpha) v_j + (*beta) v_jm1 (v_jm1, beta may be NULL) -- create_dataset.py:40,49 */
+int mlpcg_lanczos_update(double* w, const double* v_j, const double* v_jm1, const double* alpha,
+                         const double* beta, long n, void* stream);
+/* out = w / sqrt(*nrm2); *norm_out = sqrt(*nrm2) if norm_out -- create_dataset.py:42-46 */
+int mlpcg_normalize_f64(double* out, const double* w, const double* nrm2, double* norm_out, long n,
+                        void* stream);
+
 /* ---- preconditioner plan (SmallSMModelDn3D.eval_forward with K cached) ------------
  * weights/biases: host arrays of device pointers in state_dict order:
  *   SMALL_SM : pre.0..n-1, post.0..n-1, l, c0.0..n-1, c1.0..n-1      (4n+1)

@@ -68,6 +68,10 @@ SIGNATURES = {
     "mlpcg_spmv_csr": (i32, [vp, vp, vp, vp, vp, i64, i32, vp]),
     "mlpcg_dot_workspace_size": (sz, []),
     "mlpcg_dot": (i32, [vp, vp, i64, vp, vp, vp]),
+    "mlpcg_dot_f64_workspace_size": (sz, []),
+    "mlpcg_dot_f64": (i32, [vp, vp, i64, vp, vp, vp]),
+    "mlpcg_lanczos_update": (i32, [vp, vp, vp, vp, vp, i64, vp]),
+    "mlpcg_normalize_f64": (i32, [vp, vp, vp, vp, i64, vp]),
     "mlpcg_precond_create": (i32, [C.POINTER(vp), i32, i32, i32, i32, i32, i32, i32, i32, i32, i32,
                                    vp, C.POINTER(vp), C.POINTER(vp), vp]),
     "mlpcg_precond_update": (i32, [vp, vp, C.POINTER(vp), C.POINTER(vp), vp]),

@@ -20,8 +20,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmlpcg_hip.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["stencil.hip", "fused.hip", "tail.hip", "reduce.hip", "pcg.hip", "misc.hip", "precond.cpp",
-           "solver.cpp", "capi.cpp"]
+SOURCES = ["stencil.hip", "fused.hip", "tail.hip", "reduce.hip", "pcg.hip", "lanczos.hip", "misc.hip",
+           "precond.cpp", "solver.cpp", "capi.cpp"]
 
 ARCH = os.environ.get("MLPCG_OFFLOAD_ARCH", "gfx950")
 

@@ -281,6 +281,30 @@ int mlpcg_dot(const float* x, const float* y, long n, double* out, void* workspa
   return launch_dot(x, y, n, (double*)workspace, out, STREAM(stream));
 }
 
+// --------------------------------------------------------------------------- Lanczos (FP64)
+size_t mlpcg_dot_f64_workspace_size(void) { return dot_f64_workspace_bytes(); }
+
+int mlpcg_dot_f64(const double* x, const double* y, long n, double* out, void* workspace, void* stream) {
+  MLPCG_REQUIRE(x && y && out && workspace, "dot_f64: null pointer");
+  MLPCG_REQUIRE(aligned16(x) && aligned16(y), "dot_f64: vectors must be 16-byte aligned");
+  return launch_dot_f64(x, y, n, out, (double*)workspace, STREAM(stream));
+}
+
+int mlpcg_lanczos_update(double* w, const double* v_j, const double* v_jm1, const double* alpha,
+                         const double* beta, long n, void* stream) {
+  MLPCG_REQUIRE(w && v_j && alpha, "lanczos_update: null pointer");
+  MLPCG_REQUIRE(!v_jm1 || beta, "lanczos_update: v_jm1 needs beta");
+  MLPCG_REQUIRE(n >= 0, "lanczos_update: negative length");
+  return launch_lanczos_update(w, v_j, v_jm1, alpha, beta, n, STREAM(stream));
+}
+
+int mlpcg_normalize_f64(double* out, const double* w, const double* nrm2, double* norm_out, long n,
+                        void* stream) {
+  MLPCG_REQUIRE(out && w && nrm2, "normalize_f64: null pointer");
+  MLPCG_REQUIRE(n >= 0, "normalize_f64: negative length");
+  return launch_normalize_f64(out, w, nrm2, norm_out, n, STREAM(stream));
+}
+
 // --------------------------------------------------------------------------- plan
 int mlpcg_precond_create(mlpcg_precond** out, int kind, int n_levels, int C, int n1, int n2, int n3,
                          int dtype, int nearest, int mask, int swap, const void* image,

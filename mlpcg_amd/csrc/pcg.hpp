@@ -90,4 +90,12 @@ int launch_spmv_flat(const uint8_t* code, const float* x, float* y, int n1, int 
 int launch_dot(const float* x, const float* y, long n, double* partials,
                double* out, hipStream_t st);
 
+// Lanczos vector kernels (lanczos.hip), FP64
+size_t dot_f64_workspace_bytes();
+int launch_dot_f64(const double* a, const double* b, long n, double* out, double* partials, hipStream_t st);
+int launch_lanczos_update(double* w, const double* v1, const double* v0, const double* alpha,
+                          const double* beta, long n, hipStream_t st);
+int launch_normalize_f64(double* out, const double* w, const double* nrm2, double* norm_out, long n,
+                         hipStream_t st);
+
 }  // namespace mlpcg

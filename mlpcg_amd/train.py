@@ -273,7 +273,10 @@ def main(argv=None):
     ap.add_argument("--max-minutes", type=float, default=15.0)
     ap.add_argument("--init", default=None)
     ap.add_argument("--loss", choices=("relsq", "resid"), default="relsq")
-    ap.add_argument("--rhs", choices=("mix", "white", "multiscale"), default="mix")
+    ap.add_argument("--rhs", choices=("mix", "white", "multiscale", "ritz"), default="mix",
+                    help="ritz: combinations of each scene's Ritz vectors (preprocess.py:38-58)")
+    ap.add_argument("--ritz-vectors", type=int, default=200,
+                    help="Lanczos steps per scene for --rhs ritz (the reference uses 1600)")
     ap.add_argument("--select", default="npcg", help="method whose iteration count selects the kept weights")
     ap.add_argument("--model", choices=tuple(MODELS), default="sm")
     ap.add_argument("--act-dtype", choices=("fp32", "bf16"), default="fp32",
@@ -295,6 +298,15 @@ def main(argv=None):
               for s in make_scenes(args.n, range(100, 100 + args.scenes), args.kind)]
     held = SceneBatch(make_scenes(args.n, [7], args.kind)[0], args.num_imgs, dev)
     gen = torch.Generator(device=dev).manual_seed(args.seed)
+    ritz = {}
+    if args.rhs == "ritz":  # per-scene Ritz bases, as lib/create_dataset.py prepares them offline
+        from . import ritz as RZ
+        for i, sb in enumerate(scenes):
+            A = S.build_matrix(sb.scene.flags)
+            b0 = S.make_rhs(sb.scene.flags, 100 + i)[0]
+            _, Rv = RZ.ritz_vectors(RZ.CSRMatrix64(A, dev), b0, args.ritz_vectors)
+            ritz[id(sb)] = (Rv, torch.as_tensor(sb.scene.fluid_cells, device=dev))
+            print(f"scene {i}: {Rv.shape[0]} Ritz vectors", flush=True)
     t0 = time.time()
     log = []
     best = None
@@ -303,6 +315,12 @@ def main(argv=None):
         bs = [scenes[i] for i in idx]
         rhs = []
         for sb in bs:
+            if args.rhs == "ritz":
+                from . import ritz as RZ
+                Rv, fidx = ritz[id(sb)]
+                b = RZ.training_rhs(Rv, args.rhs_per_scene, num_requested=args.ritz_vectors, generator=gen)
+                rhs.append(RZ.full_grid(b, fidx, sb.fluid.numel()))
+                continue
             h = {"mix": args.rhs_per_scene // 2, "white": 0, "multiscale": args.rhs_per_scene}[args.rhs]
             parts = []
             if args.rhs_per_scene - h:

@@ -22,8 +22,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "mlpcg_amd", "csrc")
 OUT = os.path.join(HERE, "_build")
-LIB_SOURCES = ["stencil.hip", "fused.hip", "tail.hip", "reduce.hip", "pcg.hip", "misc.hip", "precond.cpp",
-               "solver.cpp", "capi.cpp"]
+LIB_SOURCES = ["stencil.hip", "fused.hip", "tail.hip", "reduce.hip", "pcg.hip", "lanczos.hip", "misc.hip",
+               "precond.cpp", "solver.cpp", "capi.cpp"]
 HARNESS_SOURCES = ["fake_kernels.cpp", "fake_hip.cpp", "harness.cpp"]
 
 

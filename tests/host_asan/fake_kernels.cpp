@@ -341,6 +341,51 @@ int launch_dot(const float* x, const float* y, long n, double* partials, double*
   return 0;
 }
 
+// ------------------------------------------------------------------ Lanczos (FP64)
+int launch_dot_f64(const double* a, const double* b, long n, double* out, double* partials, hipStream_t st) {
+  fakehip::enqueue(st, [=] {
+    const char* who = "dot_f64";
+    RD(a, 8 * n);
+    RD(b, 8 * n);
+    long nb = (n / 8 + 255) / 256;
+    nb = nb < 1 ? 1 : (nb > 8192 ? 8192 : nb);
+    if ((size_t)nb * 8 > dot_f64_workspace_bytes()) std::abort();
+    WR(partials, 8 * nb);
+    WR(out, 8);
+    double s = 0.0;
+    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    *out = s;
+  });
+  return 0;
+}
+
+int launch_lanczos_update(double* w, const double* v1, const double* v0, const double* alpha,
+                          const double* beta, long n, hipStream_t st) {
+  fakehip::enqueue(st, [=] {
+    const char* who = "lanczos_update";
+    WR(w, 8 * n);
+    RD(v1, 8 * n);
+    RD(alpha, 8);
+    if (v0) {
+      RD(v0, 8 * n);
+      RD(beta, 8);
+    }
+  });
+  return 0;
+}
+
+int launch_normalize_f64(double* out, const double* w, const double* nrm2, double* norm_out, long n,
+                         hipStream_t st) {
+  fakehip::enqueue(st, [=] {
+    const char* who = "normalize_f64";
+    WR(out, 8 * n);
+    RD(w, 8 * n);
+    RD(nrm2, 8);
+    if (norm_out) WR(norm_out, 8);
+  });
+  return 0;
+}
+
 // ------------------------------------------------------------------ preconditioner side
 int launch_avgpool2(int dtype, const void* in, void* out, int B, int n1, int n2, int n3, const int* done,
                     hipStream_t st) {

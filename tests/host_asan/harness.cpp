@@ -397,6 +397,26 @@ void block_cases() {
   void* dws = dalloc(mlpcg_dot_workspace_size());
   double* out = (double*)dalloc(8);
   OK(mlpcg_dot(xv, yv, sc.V, out, dws, nullptr));
+  // Lanczos vector kernels: a CSR FP64 product, dots, the three-term update, normalising
+  {
+    const long n = sc.nf;
+    double* vd = (double*)dalloc(8 * sc.val.size());
+    double* a = (double*)dalloc(8 * n);
+    double* b = (double*)dalloc(8 * n);
+    double* w = (double*)dalloc(8 * n);
+    double* s = (double*)dalloc(4 * 8);
+    void* fws = dalloc(mlpcg_dot_f64_workspace_size());
+    OK(mlpcg_spmv_csr(sc.d_rowptr, sc.d_col, vd, a, w, n, MLPCG_F64, nullptr));
+    OK(mlpcg_dot_f64(w, a, n, s, fws, nullptr));
+    OK(mlpcg_lanczos_update(w, a, nullptr, s, nullptr, n, nullptr));
+    OK(mlpcg_lanczos_update(w, a, b, s, s + 1, n, nullptr));
+    OK(mlpcg_dot_f64(w, w, n, s + 2, fws, nullptr));
+    OK(mlpcg_normalize_f64(b, w, s + 2, s + 3, n, nullptr));
+    OK(mlpcg_normalize_f64(b, w, s + 2, nullptr, n, nullptr));
+    EXPECT(mlpcg_dot_f64(w, a, n, nullptr, fws, nullptr) != 0, "dot_f64 accepted a null output");
+    EXPECT(mlpcg_lanczos_update(w, a, b, s, nullptr, n, nullptr) != 0, "update accepted v0 without beta");
+    for (void* p : {(void*)vd, (void*)a, (void*)b, (void*)w, (void*)s, fws}) dfree(p);
+  }
   for (void* p : {(void*)code, (void*)xv, (void*)yv, dws, (void*)out}) dfree(p);
 }
 

@@ -102,6 +102,18 @@ int mlpcg_sm_linear_inference(const void* image, const void* weight, const void*
 int mlpcg_sm_linear_forward(const void* image, const void* weight, const void* bias, void* z,
                             void* ybar, void* workspace, int C, int n1, int n2, int n3, int dtype,
                             void* stream);
+/* No counterpart in the reference: mlpcg_sm_linear_inference for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same results as mlpcg_sm_linear_inference (same kernels, same
+ * order); the reduction partials live in a device array of the library,
+ * mlpcg_sm_linear_noalloc_scratch_bytes() bytes per GPU (648 KiB: at most 1024
+ * workgroups x 27 C doubles, which covers every grid), held until the process ends.
+ * Calls that overlap on one GPU (concurrent streams or threads) share that array and race:
+ * serialise them.  Grids whose reduction would not fit are refused with an error. */
+size_t mlpcg_sm_linear_noalloc_scratch_bytes(void);
+int mlpcg_sm_linear_inference_noalloc(const void* image, const void* weight, const void* bias, void* z,
+                                      void* ybar, int C, int n1, int n2, int n3, int dtype,
+                                      void* stream);
 /* sm_linear_3d.cpp:47  smlinear3d.backward(grad_output, ybar) -> [gw, gb] */
 int mlpcg_sm_linear_backward(const void* grad_z, const void* ybar, void* grad_w, void* grad_b,
                              int C, int dtype, void* stream);

@@ -57,6 +57,8 @@ SIGNATURES = {
     "mlpcg_sm_linear_inference": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp]),
     "mlpcg_sm_linear_forward": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp]),
     "mlpcg_sm_linear_backward": (i32, [vp, vp, vp, vp, i32, i32, vp]),
+    "mlpcg_sm_linear_noalloc_scratch_bytes": (sz, []),
+    "mlpcg_sm_linear_inference_noalloc": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, vp]),
     "mlpcg_stencil_weights": (i32, [vp, i32, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
     "mlpcg_sm_apply": (i32, [vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32,
                              vp]),

@@ -169,6 +169,14 @@ int mlpcg_sm_linear_inference(const void* image, const void* weight, const void*
   MLPCG_REQUIRE(aligned16(image), "sm_linear: image must be 16-byte aligned");
   return launch_sm_linear(dtype, image, weight, bias, z, ybar, workspace, C, n1, n2, n3, STREAM(stream));
 }
+size_t mlpcg_sm_linear_noalloc_scratch_bytes(void) { return sm_linear_scratch_bytes(); }
+int mlpcg_sm_linear_inference_noalloc(const void* image, const void* weight, const void* bias, void* z,
+                                      void* ybar, int C, int n1, int n2, int n3, int dtype, void* stream) {
+  if (check_block_args("sm_linear_noalloc", C, 1, n1, n2, n3, dtype)) return -1;
+  MLPCG_REQUIRE(image && weight && bias && z && ybar, "sm_linear_noalloc: null pointer");
+  MLPCG_REQUIRE(aligned16(image), "sm_linear_noalloc: image must be 16-byte aligned");
+  return launch_sm_linear_noalloc(dtype, image, weight, bias, z, ybar, C, n1, n2, n3, STREAM(stream));
+}
 int mlpcg_sm_linear_forward(const void* image, const void* weight, const void* bias, void* z,
                             void* ybar, void* workspace, int C, int n1, int n2, int n3, int dtype,
                             void* stream) {

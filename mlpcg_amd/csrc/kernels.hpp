@@ -105,6 +105,11 @@ int launch_upsample2_adjoint(int dtype, const void* gin, void* gout, int B, int 
 size_t mean_patch_workspace(int dtype, int C, int n1, int n2, int n3);
 int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,
                      void* workspace, int C, int n1, int n2, int n3, hipStream_t st);
+// The same with the partials in the library's own device array (no allocation, no
+// workspace argument); bytes of that array per GPU.
+int launch_sm_linear_noalloc(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,
+                             int C, int n1, int n2, int n3, hipStream_t st);
+size_t sm_linear_scratch_bytes();
 
 // Weight gradient of the SM block: gW[p,s,e] = sum_{b,v} a[b,v] c[b,v+d_p] img[s,v+e],
 // gb[p] = sum_{b,v} a[b,v] c[b,v+d_p]; (a, c) = (grad_out, x) for the block and

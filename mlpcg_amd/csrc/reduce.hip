@@ -8,6 +8,8 @@
 //               gW[p,s,e] = sum_v G_p(v) img[s,v+e], gb[p] = sum_v G_p(v)
 //               i.e. the [27 x V] . [V x (27C+1)] product of per-voxel outer products,
 //               tiled through LDS with a 3x4 register tile per thread.
+#include <mutex>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -137,6 +139,54 @@ int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, v
     mean_patch_final_kernel<float><<<1, kThreads, 0, st>>>(part, nblk, 27 * C, invV, (const float*)w,
                                                            (const float*)b, (float*)z, (float*)ybar);
   return check_launch("sm_linear");
+}
+
+// ---------------------------------------------------------------------------
+// SM linear without allocation.  The block partials of the mean patch go to a file-scope
+// device array instead of a caller workspace: at most 1024 blocks x 27 C doubles
+// (648 KiB for C = 3) for ANY grid, since the partial kernel caps its block count.
+// Each GPU holds its own copy of the array until the process ends; host code never
+// passes the array's name to a kernel (in host code it names a host-side stand-in) but
+// the device address that hipGetSymbolAddress returned, fetched once per device with
+// that device current.  Calls that overlap on one GPU share the array and race; nothing
+// in it needs clearing (every launched block writes all of its partials before the
+// final pass reads them).
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)1024 * 27 * 3 * sizeof(double);
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+size_t sm_linear_scratch_bytes() { return kScratchBytes; }
+
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  MLPCG_HIP(hipGetDevice(&dev));
+  MLPCG_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "sm_linear_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MLPCG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MLPCG_REQUIRE(p != nullptr, "sm_linear_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+int launch_sm_linear_noalloc(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,
+                             int C, int n1, int n2, int n3, hipStream_t st) {
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "sm_linear_noalloc: C must be 1, 2 or 3 (got %d)", C);
+  MLPCG_REQUIRE(n1 >= 1 && n2 >= 1 && n3 >= 1, "sm_linear_noalloc: bad grid (%d,%d,%d)", n1, n2, n3);
+  const size_t need = mean_patch_workspace(dtype, C, n1, n2, n3);
+  MLPCG_REQUIRE(need <= kScratchBytes,
+                "sm_linear_noalloc: the reduction needs %zu bytes of scratch, more than the %zu reserved", need,
+                kScratchBytes);
+  void* ws = nullptr;
+  if (scratch_for_current_device(&ws)) return -1;
+  return launch_sm_linear(dtype, img, w, b, z, ybar, ws, C, n1, n2, n3, st);
 }
 
 // gw = g * ybar, gb = g  (sm_linear_3d_kernel.cu:229-238)

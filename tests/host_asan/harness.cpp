@@ -365,6 +365,10 @@ void block_cases() {
         OK(mlpcg_sm_linear_inference(img, lw, lb, z, ybar, lws, C, g1, g2, g3, dtype, nullptr));
         OK(mlpcg_sm_linear_forward(img, lw, lb, z, ybar, lws, C, g1, g2, g3, dtype, nullptr));
         OK(mlpcg_sm_linear_backward(z, ybar, lw, lb, C, dtype, nullptr));
+        OK(mlpcg_sm_linear_inference_noalloc(img, lw, lb, z, ybar, C, g1, g2, g3, dtype, nullptr));
+        OK(mlpcg_sm_linear_inference_noalloc(img, lw, lb, z, ybar, C, g1 / 2, g2, g3 / 2, dtype, nullptr));
+        EXPECT(mlpcg_sm_linear_inference_noalloc(img, lw, lb, z, ybar, 4, g1, g2, g3, dtype, nullptr) != 0,
+               "noalloc accepted C = 4");
         for (void* p : {img, x, y, g, gx, W, bias, gW, gb, ws, lw, lb, z, ybar, lws}) dfree(p);
       }
   // streamed-stencil building blocks

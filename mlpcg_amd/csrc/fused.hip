@@ -168,7 +168,14 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
 // places y(n) of group t in lane (g, n) with g = (n/4 + t) mod 4, element n % 4: over the
 // 4 groups of a 64-voxel chunk every lane ends up holding exactly one voxel, voxel n of
 // group t* = (g - n/4) mod 4, i.e. z0 + 32 (t*/2) + 2n + t* % 2.
-template <typename TX, typename TY, typename TP, int C, int FI>
+//
+// EPI selects the epilogue at compile time (kEpiRuntime: every operand tested at run time):
+// each present operand costs its per-chunk load and its VALU work, so the hot launches of
+// a solve (pre blocks: none; post blocks: prev, and on level 0 the output mask and the
+// solver's dot products) run without the others' dead loads, selects and FP64 FMAs.
+constexpr int kEpiPrev = 1, kEpiMaskA = 2, kEpiMaskB = 4, kEpiDot0 = 8, kEpiDot1 = 16;
+constexpr int kEpiRuntime = -1;
+template <typename TX, typename TY, typename TP, int C, int FI, int EPI>
 __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   constexpr int kTile = tile_cells(FI);
   constexpr int kRows = (FI + 2) * kPj / 4;  // halo-tile rows staged per wave
@@ -282,11 +289,17 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   // loads are unconditional and can be issued one chunk ahead.  Row pointers are uniform;
   // the lane's element offset myz is the same for every chunk.
   TY* __restrict__ y = (TY*)a.y;
+  // LX: the operand is loaded (a compile-time epilogue has it, or the run-time one may);
+  // has_x: it is applied
+  constexpr bool RT = EPI == kEpiRuntime;
+  constexpr bool LA = RT || (EPI & kEpiMaskA), LB = RT || (EPI & kEpiMaskB), LP = RT || (EPI & kEpiPrev);
+  constexpr bool LD0 = RT || (EPI & kEpiDot0), LD1 = RT || (EPI & kEpiDot1);
   const uint8_t* mA = a.mask_a ? a.mask_a : (const uint8_t*)a.x;
   const uint8_t* mB = a.mask_b ? a.mask_b : (const uint8_t*)a.x;
   const TP* pP = a.prev ? (const TP*)a.prev : (const TP*)a.x;
-  const bool has_a = a.mask_a != nullptr, has_b = a.mask_b != nullptr, has_p = a.prev != nullptr;
-  const bool dots = a.partials != nullptr;
+  const bool has_a = RT ? a.mask_a != nullptr : LA, has_b = RT ? a.mask_b != nullptr : LB;
+  const bool has_p = RT ? a.prev != nullptr : LP;
+  const bool dots = RT ? a.partials != nullptr : LD0;
   const float* d0p = a.dot0 ? a.dot0 : (const float*)a.x;
   const float* d1p = a.dot1 ? a.dot1 : (const float*)a.x;
   const long rstep1 = a.dot1 ? 1 : 0;               // without dot1 every lane reads element 0
@@ -331,18 +344,23 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   };
   Words cur = load_words(cb_of(0), 0);
   long vr = row(0);
-  uint8_t ka = lane_at(mA + vr, mob), kb = lane_at(mB + vr, mob);
-  TP pv = lane_at(pP + vr, mop);
-  float da = dots ? lane_at(d0p + vr, mof) : 0.f, db = dots ? lane_at(d1p + vr * rstep1, d1off) : 0.f;
+  uint8_t ka = LA ? lane_at(mA + vr, mob) : uint8_t(1), kb = LB ? lane_at(mB + vr, mob) : uint8_t(1);
+  TP pv = LP ? lane_at(pP + vr, mop) : TP(0.f);
+  float da = (LD0 && dots) ? lane_at(d0p + vr, mof) : 0.f;
+  float db = (LD1 && dots) ? lane_at(d1p + vr * rstep1, d1off) : 0.f;
+  // two chunks per trip: the odd chunk's LDS words sit 4 tile rows (528 bytes) past the
+  // even chunk's, within the read2 immediate range, so both share the lane addresses
+#pragma unroll 2
   for (int q = 0; q < FI * 2; ++q) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
     const int cb = ((ii + 1) * kPj + (jj + 1)) * kPz + 1;
     // next chunk's epilogue operands
     const long vn = row(q + 1 < FI * 2 ? q + 1 : q);
-    const uint8_t ka_n = lane_at(mA + vn, mob), kb_n = lane_at(mB + vn, mob);
-    const TP pv_n = lane_at(pP + vn, mop);
-    const float da_n = dots ? lane_at(d0p + vn, mof) : 0.f;
-    const float db_n = dots ? lane_at(d1p + vn * rstep1, d1off) : 0.f;
+    const uint8_t ka_n = LA ? lane_at(mA + vn, mob) : uint8_t(1);
+    const uint8_t kb_n = LB ? lane_at(mB + vn, mob) : uint8_t(1);
+    const TP pv_n = LP ? lane_at(pP + vn, mop) : TP(0.f);
+    const float da_n = (LD0 && dots) ? lane_at(d0p + vn, mof) : 0.f;
+    const float db_n = (LD1 && dots) ? lane_at(d1p + vn * rstep1, d1off) : 0.f;
     f32x4 c2[4];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -396,7 +414,7 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     lane_store(y + vr, moy, yv);
     if (dots) {
       red[0] += (double)(float)yv * (double)da;
-      red[1] += (double)(float)yv * (double)db;
+      if (LD1) red[1] += (double)(float)yv * (double)db;
     }
     vr = vn;
     ka = ka_n;
@@ -433,24 +451,50 @@ int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, h
   return check_launch("fused_pack");
 }
 
-template <typename TX, typename TY, int C>
+template <typename TX, typename TY, int C, int EPI>
 static void launch_fused_t(const FusedArgs& a, hipStream_t st) {
   const unsigned nb = fused_blocks(a.n1, a.n2, a.n3);
   const int fi = fused_fi(a.n1, a.n2, a.n3);
-  if (fi == 8) sm_fused_kernel<TX, TY, bf16, C, 8><<<nb, kThreads, 0, st>>>(a);
-  else if (fi == 4) sm_fused_kernel<TX, TY, bf16, C, 4><<<nb, kThreads, 0, st>>>(a);
-  else sm_fused_kernel<TX, TY, bf16, C, 2><<<nb, kThreads, 0, st>>>(a);
+  if (fi == 8) sm_fused_kernel<TX, TY, bf16, C, 8, EPI><<<nb, kThreads, 0, st>>>(a);
+  else if (fi == 4) sm_fused_kernel<TX, TY, bf16, C, 4, EPI><<<nb, kThreads, 0, st>>>(a);
+  else sm_fused_kernel<TX, TY, bf16, C, 2, EPI><<<nb, kThreads, 0, st>>>(a);
+}
+
+// The epilogues of a preconditioner application get compile-time kernels: the pre blocks
+// and the lone-block apply (none; FP32 -> BF16 on level 0, BF16 -> BF16 below), the
+// post blocks below level 0 (prev) and level 0 (prev, the output mask and none, one or
+// two solver dot products; BF16 -> FP32).  Anything else (the model's mask, other type
+// pairs) runs the run-time epilogue.
+template <typename TX, typename TY, int C>
+static void launch_fused_epi(const FusedArgs& a, hipStream_t st) {
+  const int epi = (a.prev ? kEpiPrev : 0) | (a.mask_a ? kEpiMaskA : 0) | (a.mask_b ? kEpiMaskB : 0) |
+                  (a.partials ? kEpiDot0 : 0) | (a.partials && a.dot1 ? kEpiDot1 : 0);
+  if constexpr (sizeof(TY) == 2) {
+    if (epi == 0) return launch_fused_t<TX, TY, C, 0>(a, st);
+    if constexpr (sizeof(TX) == 2)
+      if (epi == kEpiPrev) return launch_fused_t<TX, TY, C, kEpiPrev>(a, st);
+  } else {
+    if (epi == 0) return launch_fused_t<TX, TY, C, 0>(a, st);
+    if (epi == (kEpiPrev | kEpiMaskB)) return launch_fused_t<TX, TY, C, kEpiPrev | kEpiMaskB>(a, st);
+    if (epi == (kEpiPrev | kEpiMaskB | kEpiDot0))
+      return launch_fused_t<TX, TY, C, kEpiPrev | kEpiMaskB | kEpiDot0>(a, st);
+    if (epi == (kEpiPrev | kEpiMaskB | kEpiDot0 | kEpiDot1))
+      return launch_fused_t<TX, TY, C, kEpiPrev | kEpiMaskB | kEpiDot0 | kEpiDot1>(a, st);
+  }
+  launch_fused_t<TX, TY, C, kEpiRuntime>(a, st);
 }
 
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& args, hipStream_t st) {
   MLPCG_REQUIRE(tw == kBF16, "sm_fused: BF16 plans only (got %d)", tw);
   MLPCG_REQUIRE(fused_eligible(C, 1, args.n1, args.n2, args.n3),
                 "sm_fused: grid (%d,%d,%d) / C=%d not supported", args.n1, args.n2, args.n3, C);
+  MLPCG_REQUIRE(!args.dot1 || args.dot0, "sm_fused: dot1 needs dot0");
+  MLPCG_REQUIRE(!args.partials || args.dot0, "sm_fused: partials need dot0");
   const FusedArgs& a = args;
 #define MLPCG_FU(TXX, TYY)                                 \
   do {                                                     \
-    if (C == 1) launch_fused_t<TXX, TYY, 1>(a, st);        \
-    else launch_fused_t<TXX, TYY, 2>(a, st);               \
+    if (C == 1) launch_fused_epi<TXX, TYY, 1>(a, st);      \
+    else launch_fused_epi<TXX, TYY, 2>(a, st);             \
   } while (0)
   if (tx == kF32 && ty == kBF16) MLPCG_FU(float, bf16);
   else if (tx == kBF16 && ty == kF32) MLPCG_FU(bf16, float);

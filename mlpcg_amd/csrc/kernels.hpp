@@ -105,6 +105,11 @@ int launch_upsample2_adjoint(int dtype, const void* gin, void* gout, int B, int 
 size_t mean_patch_workspace(int dtype, int C, int n1, int n2, int n3);
 int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,
                      void* workspace, int C, int n1, int n2, int n3, hipStream_t st);
+// Two SM linears on one image (one mean patch): z0 = w0 . ybar + b0, z1 = w1 . ybar + b1
+// (w1 may be null: z1 untouched).
+int launch_sm_linear_pair(int dtype, const void* img, const void* w0, const void* b0, void* z0,
+                          const void* w1, const void* b1, void* z1, void* ybar, void* workspace, int C,
+                          int n1, int n2, int n3, hipStream_t st);
 // The same with the partials in the library's own device array (no allocation, no
 // workspace argument); bytes of that array per GPU.
 int launch_sm_linear_noalloc(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,

@@ -390,9 +390,10 @@ __device__ __forceinline__ float dir_at(const DirIn& d, long v) {
   return (float)u;
 }
 
+// Q at VZ consecutive voxels; nv receives N there (0 without N)
 template <int VZ>
-__device__ __forceinline__ void dir_vec(const DirIn& d, long v0, float (&out)[VZ]) {
-  float qv[VZ], nv[VZ], ov[VZ];
+__device__ __forceinline__ void dir_vec(const DirIn& d, long v0, float (&out)[VZ], float (&nv)[VZ]) {
+  float qv[VZ], ov[VZ];
   load_vec<VZ>(d.q + v0, qv);
   if (d.pN) load_vec<VZ>(d.pN + v0, nv);
   if (d.pO) load_vec<VZ>(d.pO + v0, ov);
@@ -400,19 +401,22 @@ __device__ __forceinline__ void dir_vec(const DirIn& d, long v0, float (&out)[VZ
   for (int t = 0; t < VZ; ++t) {
     double u = (double)qv[t];
     if (d.pN) u = fma(-d.gN, (double)nv[t], u);
+    else nv[t] = 0.f;
     if (d.pO) u = fma(-d.gO, (double)ov[t], u);
     out[t] = (float)u;
   }
 }
 
-// Q on the 7-point neighbourhood of a run (layout of load_star).
+// Q on the 7-point neighbourhood of a run (layout of load_star); ncen receives N on the
+// run itself.
 template <int VZ>
 __device__ __forceinline__ void load_dir_star(const DirIn& d, const Run<VZ>& r, int n1, int n2,
-                                              int n3, float (&c)[VZ + 2], float (&s)[4][VZ]) {
+                                              int n3, float (&c)[VZ + 2], float (&s)[4][VZ],
+                                              float (&ncen)[VZ]) {
   const long s1 = (long)n2 * n3;
   const int lane = threadIdx.x & 63;
   float cv[VZ];
-  dir_vec<VZ>(d, r.v0, cv);
+  dir_vec<VZ>(d, r.v0, cv, ncen);
 #pragma unroll
   for (int t = 0; t < VZ; ++t) c[t + 1] = cv[t];
   float lft = wave_from_prev(c[VZ]), rgt = wave_from_next(c[1]);  // see load_star
@@ -425,7 +429,8 @@ __device__ __forceinline__ void load_dir_star(const DirIn& d, const Run<VZ>& r, 
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (ok[q]) {
-      dir_vec<VZ>(d, r.v0 + offs[q], s[q]);
+      float nside[VZ];
+      dir_vec<VZ>(d, r.v0 + offs[q], s[q], nside);
     } else {
 #pragma unroll
       for (int t = 0; t < VZ; ++t) s[q][t] = 0.f;
@@ -444,11 +449,12 @@ __device__ __forceinline__ DirIn dir_in(const float* q, const float* pN, const f
   return d;
 }
 
-// Q -> a.p_new, A Q -> a.w; reduces [Q.AQ, Q.r, Q.AN] (AN = a.p, may be null).
-// a.z = q, a.p_old = O; N comes in as pN.
+// Q -> a.p_new, A Q -> a.w; reduces [Q.AQ, Q.r, N.AQ] (the cross term Q.AN of the next
+// projection, formed as N.AQ: A is symmetric on the unknowns and Q, N, AQ, AN vanish off
+// them, so the two are equal, and N.AQ needs no read of AN).  a.z = q, a.p_old = O; N
+// comes in as pN (may be null).
 template <int VZ>
-__global__ __launch_bounds__(kThreads) void npsd_dir_flat_kernel(FlatArgs a, const float* pN,
-                                                                 const float* aN) {
+__global__ __launch_bounds__(kThreads) void npsd_dir_flat_kernel(FlatArgs a, const float* pN) {
   SolveState* st = a.st;
   if (st->done) return;
   const DirIn d = dir_in(a.z, pN, pN ? a.p_old : nullptr, st);
@@ -458,12 +464,11 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_flat_kernel(FlatArgs a, con
        cid += (long)gridDim.x * kThreads) {
     Run<VZ> r;
     run_at<VZ>(cid, a.n1, a.n2, a.n3, r);
-    float c[VZ + 2], s[4][VZ];
-    load_dir_star<VZ>(d, r, a.n1, a.n2, a.n3, c, s);
-    float code[VZ], rv[VZ], wv[VZ], qn[VZ], av[VZ];
+    float c[VZ + 2], s[4][VZ], nv[VZ];
+    load_dir_star<VZ>(d, r, a.n1, a.n2, a.n3, c, s, nv);
+    float code[VZ], rv[VZ], wv[VZ], qn[VZ];
     load_vec<VZ>(a.code + r.v0, code);
     load_vec<VZ>(a.r + r.v0, rv);
-    if (aN) load_vec<VZ>(aN + r.v0, av);
 #pragma unroll
     for (int t = 0; t < VZ; ++t) {
       const int cd = (int)code[t];
@@ -471,7 +476,7 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_flat_kernel(FlatArgs a, con
       wv[t] = cd ? lap_at<VZ, float>(c, s, t, cd) : 0.f;
       red[0] += (double)qn[t] * (double)wv[t];
       red[1] += (double)qn[t] * (double)rv[t];
-      if (aN) red[2] += (double)qn[t] * (double)av[t];
+      if (pN) red[2] += (double)nv[t] * (double)wv[t];
     }
     store_vec<VZ>(a.p_new + r.v0, qn);
     store_vec<VZ>(a.w + r.v0, wv);
@@ -479,8 +484,7 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_flat_kernel(FlatArgs a, con
   block_partials<3>(red, a.partials);
 }
 
-__global__ __launch_bounds__(kThreads) void npsd_dir_csr_kernel(CsrArgs a, const float* pN,
-                                                                const float* aN) {
+__global__ __launch_bounds__(kThreads) void npsd_dir_csr_kernel(CsrArgs a, const float* pN) {
   SolveState* st = a.st;
   if (st->done) return;
   const DirIn d = dir_in(a.z, pN, pN ? a.p_old : nullptr, st);
@@ -494,7 +498,7 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_csr_kernel(CsrArgs a, const
     a.w[row] = w;
     red[0] += (double)qn * (double)w;
     red[1] += (double)qn * (double)a.r[row];
-    if (aN) red[2] += (double)qn * (double)aN[row];
+    if (pN) red[2] += (double)pN[row] * (double)w;  // N.AQ = Q.AN, as in the flat kernel
   }
   block_partials<3>(red, a.partials);
 }
@@ -642,9 +646,9 @@ int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st) {
   return check_launch("p_spmv_flat");
 }
 
-int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, const float* aN, double* alfa_out,
+int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, double* alfa_out,
                          hipStream_t st) {
-  MLPCG_FLAT_DISPATCH(npsd_dir_flat_kernel, 0, a, pN, aN);
+  MLPCG_FLAT_DISPATCH(npsd_dir_flat_kernel, 0, a, pN);
   finalize(kFinNpsd, 3, a.partials, flat_blocks(a), a.st, alfa_out, st);
   return check_launch("npsd_dir_flat");
 }
@@ -683,10 +687,10 @@ int launch_resid_csr(const CsrArgs& a, int init, hipStream_t st) {
   return check_launch("resid_csr");
 }
 
-int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, const float* aN, double* alfa_out,
+int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, double* alfa_out,
                         hipStream_t st) {
   const unsigned nb = red_blocks(a.n);
-  npsd_dir_csr_kernel<<<nb, kThreads, 0, st>>>(a, pN, aN);
+  npsd_dir_csr_kernel<<<nb, kThreads, 0, st>>>(a, pN);
   finalize(kFinNpsd, 3, a.partials, nb, a.st, alfa_out, st);
   return check_launch("npsd_dir_csr");
 }

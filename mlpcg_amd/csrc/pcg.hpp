@@ -62,7 +62,7 @@ long reduction_capacity();
 int launch_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3, hipStream_t st);
 int launch_resid_flat(const FlatArgs& a, int init, hipStream_t st);
 int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st);
-int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, const float* aN, double* alfa_out,
+int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, double* alfa_out,
                          hipStream_t st);
 int launch_rz_dot(const float* r, const float* z, const float* r_old, long n, SolveState* s,
                   double* partials, hipStream_t st);
@@ -72,7 +72,7 @@ int launch_gather(const int* idx, const float* grid, float* dst, long n, const i
                   hipStream_t st);
 int launch_p_spmv_csr(const CsrArgs& a, hipStream_t st);
 int launch_resid_csr(const CsrArgs& a, int init, hipStream_t st);
-int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, const float* aN, double* alfa_out,
+int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, double* alfa_out,
                         hipStream_t st);
 // Finalizers for dot products computed elsewhere (the fused apply's epilogue):
 // partials [2][nb] of z.r and z.r_old (-> rho, beta), or of q.AN and q.AO (-> proj).

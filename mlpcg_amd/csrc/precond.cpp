@@ -179,15 +179,15 @@ int Precond::update(const void* image, const void* const* weights, const void* c
       TRY(launch_stencil_weights2(acc, dtype, img[b.level], weights[b.w], biases[b.w], K[k], C, d[0],
                                   d[1], d[2], b.trans != 0, st));
   }
-  // SMLinear coefficients c0[i], c1[i] from img[i] (plus their squares for kSmallSPD)
-  for (int which = 0; which < 2; ++which) {
-    for (int i = 0; i < n; ++i) {
-      const int widx = nw + which * n + i;
-      char* z = (char*)coef + (size_t)(which * n + i) * as;
-      const int* d = dims[i];
-      TRY(launch_sm_linear(acc, img[i], weights[widx], biases[widx], z, ybar, lin_ws, C, d[0], d[1], d[2],
-                           st));
-    }
+  // SMLinear coefficients c0[i], c1[i] from img[i], both from one mean patch of level i
+  // (plus their squares for kSmallSPD)
+  for (int i = 0; i < n; ++i) {
+    const int w0 = nw + i, w1 = nw + n + i;
+    char* z0 = (char*)coef + (size_t)i * as;
+    char* z1 = (char*)coef + (size_t)(n + i) * as;
+    const int* d = dims[i];
+    TRY(launch_sm_linear_pair(acc, img[i], weights[w0], biases[w0], z0, weights[w1], biases[w1], z1, ybar,
+                              lin_ws, C, d[0], d[1], d[2], st));
   }
   if (kind == kSmallSPD) TRY(launch_square(acc, coef, (char*)coef + (size_t)2 * n * as, 2 * n, st));
 #undef TRY

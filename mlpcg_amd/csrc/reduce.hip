@@ -71,23 +71,37 @@ __global__ __launch_bounds__(kThreads) void mean_patch_partial_kernel(const TI* 
     partials[(size_t)blockIdx.x * NV + o] = sh[o][0] + sh[o][1] + sh[o][2] + sh[o][3];
 }
 
+// ybar = (sum of the block partials) / V, then z0 = w0 . ybar + b0 and, if w1 is given,
+// z1 = w1 . ybar + b1: c0 and c1 of one level share their mean patch (sm_model_3d.py:236-239
+// evaluates both on the same image), so one reduction serves both.  One wave per output
+// (lanes strided over the partials, a wave sum), all outputs in flight at once.
 template <typename A>
 __global__ __launch_bounds__(kThreads) void mean_patch_final_kernel(
-    const double* __restrict__ partials, int nblk, int NV, double invV, const A* __restrict__ w,
-    const A* __restrict__ b, A* __restrict__ z, A* __restrict__ ybar) {
-  // every output is summed by the whole block (strided over the partials, then a block
-  // reduction), so the pass costs nblk/256 loads per thread per output
-  __shared__ double sh[4];
-  double contrib = 0.0;
-  for (int o = 0; o < NV; ++o) {
+    const double* __restrict__ partials, int nblk, int NV, double invV, const A* __restrict__ w0,
+    const A* __restrict__ b0, A* __restrict__ z0, const A* __restrict__ w1, const A* __restrict__ b1,
+    A* __restrict__ z1, A* __restrict__ ybar) {
+  __shared__ A ys[27 * 3];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int o = wv; o < NV; o += kThreads / 64) {
     double s = 0.0;
-    for (int blk = threadIdx.x; blk < nblk; blk += kThreads) s += partials[(size_t)blk * NV + o];
-    s = block_sum(s, sh);
+    for (int blk = lane; blk < nblk; blk += 64) s += partials[(size_t)blk * NV + o];
+    s = wave_sum(s);
     const A yb = (A)(s * invV);
-    if (threadIdx.x == 0) ybar[o] = yb;
-    contrib += (double)w[o] * (double)yb;
+    if (lane == 0) {
+      ys[o] = yb;
+      ybar[o] = yb;
+    }
   }
-  if (threadIdx.x == 0) z[0] = (A)(contrib + (double)b[0]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double c0 = 0.0, c1 = 0.0;
+    for (int o = 0; o < NV; ++o) {
+      c0 += (double)w0[o] * (double)ys[o];
+      if (w1) c1 += (double)w1[o] * (double)ys[o];
+    }
+    z0[0] = (A)(c0 + (double)b0[0]);
+    if (w1) z1[0] = (A)(c1 + (double)b1[0]);
+  }
 }
 
 static int mean_patch_blocks(int n1, int n2, int n3, int vz) {
@@ -118,9 +132,11 @@ static void launch_mp_t(const void* img, double* part, int nblk, int C, int n1, 
 #undef MLPCG_MP
 }
 
-int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,
-                     void* workspace, int C, int n1, int n2, int n3, hipStream_t st) {
+int launch_sm_linear_pair(int dtype, const void* img, const void* w0, const void* b0, void* z0,
+                          const void* w1, const void* b1, void* z1, void* ybar, void* workspace, int C,
+                          int n1, int n2, int n3, hipStream_t st) {
   MLPCG_REQUIRE(C >= 1 && C <= 3, "sm_linear: C must be 1, 2 or 3 (got %d)", C);
+  MLPCG_REQUIRE(!w1 || (b1 && z1), "sm_linear: a second weight needs its bias and output");
   const int vz = mean_patch_vz(n3);
   const int nblk = mean_patch_blocks(n1, n2, n3, vz);
   double* part = (double*)workspace;
@@ -133,12 +149,20 @@ int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, v
     default: MLPCG_REQUIRE(false, "sm_linear: bad dtype %d", dtype);
   }
   if (dtype == kF64)
-    mean_patch_final_kernel<double><<<1, kThreads, 0, st>>>(part, nblk, 27 * C, invV, (const double*)w,
-                                                            (const double*)b, (double*)z, (double*)ybar);
+    mean_patch_final_kernel<double><<<1, kThreads, 0, st>>>(
+        part, nblk, 27 * C, invV, (const double*)w0, (const double*)b0, (double*)z0, (const double*)w1,
+        (const double*)b1, (double*)z1, (double*)ybar);
   else
-    mean_patch_final_kernel<float><<<1, kThreads, 0, st>>>(part, nblk, 27 * C, invV, (const float*)w,
-                                                           (const float*)b, (float*)z, (float*)ybar);
+    mean_patch_final_kernel<float><<<1, kThreads, 0, st>>>(
+        part, nblk, 27 * C, invV, (const float*)w0, (const float*)b0, (float*)z0, (const float*)w1,
+        (const float*)b1, (float*)z1, (float*)ybar);
   return check_launch("sm_linear");
+}
+
+int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, void* z, void* ybar,
+                     void* workspace, int C, int n1, int n2, int n3, hipStream_t st) {
+  return launch_sm_linear_pair(dtype, img, w, b, z, nullptr, nullptr, nullptr, ybar, workspace, C, n1, n2, n3,
+                               st);
 }
 
 // ---------------------------------------------------------------------------

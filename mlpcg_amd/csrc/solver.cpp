@@ -277,7 +277,7 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
       fa.p_new = q_new;
       fa.r = const_cast<float*>(rk);
       fa.w = aq_hist[cnt & 1];
-      TRY2(launch_npsd_dir_flat(fa, pN, aN, &this->st->alfa[cnt & 1], s));
+      TRY2(launch_npsd_dir_flat(fa, pN, &this->st->alfa[cnt & 1], s));
       fa.x_old = x[cur];
       fa.x_new = x[nxt];
       fa.p = q_new;
@@ -289,7 +289,7 @@ int Solver::solve(Precond* P, const float* b, const float* x0, float* x_out, int
       ca.p_new = q_new;
       ca.r = const_cast<float*>(rk);
       ca.w = aq_hist[cnt & 1];
-      TRY2(launch_npsd_dir_csr(ca, pN, aN, &this->st->alfa[cnt & 1], s));
+      TRY2(launch_npsd_dir_csr(ca, pN, &this->st->alfa[cnt & 1], s));
       ca.x_old = x[cur];
       ca.x_new = x[nxt];
       ca.p = q_new;

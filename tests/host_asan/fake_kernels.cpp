@@ -137,7 +137,7 @@ int launch_p_spmv_flat(const FlatArgs& a, hipStream_t st) {
   return 0;
 }
 
-int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, const float* aN, double* alfa_out,
+int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, double* alfa_out,
                          hipStream_t st) {
   fakehip::enqueue(st, [=] {
     const char* who = "npsd_dir_flat";
@@ -148,7 +148,6 @@ int launch_npsd_dir_flat(const FlatArgs& a, const float* pN, const float* aN, do
     RD(a.r, 4 * V);
     if (pN) {
       RD(pN, 4 * V);
-      RD(aN, 4 * V);
       if (a.p_old) RD(a.p_old, 4 * V);
     }
     WR(a.p_new, 4 * V);
@@ -234,7 +233,7 @@ int launch_resid_csr(const CsrArgs& a, int init, hipStream_t st) {
   return 0;
 }
 
-int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, const float* aN, double* alfa_out,
+int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, double* alfa_out,
                         hipStream_t st) {
   fakehip::enqueue(st, [=] {
     const char* who = "npsd_dir_csr";
@@ -244,7 +243,6 @@ int launch_npsd_dir_csr(const CsrArgs& a, const float* pN, const float* aN, doub
     RD(a.r, 4 * a.n);
     if (pN) {
       RD(pN, 4 * a.n);
-      RD(aN, 4 * a.n);
       if (a.p_old) RD(a.p_old, 4 * a.n);
     }
     WR(a.p_new, 4 * a.n);
@@ -537,6 +535,32 @@ int launch_sm_linear(int dtype, const void* img, const void* w, const void* b, v
     WR(z, accsz(dtype));
     WR(ybar, 27 * C * accsz(dtype));
     std::memset(z, 0, accsz(dtype));
+  });
+  return 0;
+}
+
+int launch_sm_linear_pair(int dtype, const void* img, const void* w0, const void* b0, void* z0,
+                          const void* w1, const void* b1, void* z1, void* ybar, void* workspace, int C,
+                          int n1, int n2, int n3, hipStream_t st) {
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "sm_linear: C");
+  MLPCG_REQUIRE(!w1 || (b1 && z1), "sm_linear: pair");
+  const size_t V = (size_t)n1 * n2 * n3;
+  const size_t ws = mean_patch_workspace(dtype, C, n1, n2, n3);
+  fakehip::enqueue(st, [=] {
+    const char* who = "sm_linear_pair";
+    RD(img, C * V * esz(dtype));
+    WR(workspace, ws);
+    RD(w0, 27 * C * accsz(dtype));
+    RD(b0, accsz(dtype));
+    WR(z0, accsz(dtype));
+    if (w1) {
+      RD(w1, 27 * C * accsz(dtype));
+      RD(b1, accsz(dtype));
+      WR(z1, accsz(dtype));
+      std::memset(z1, 0, accsz(dtype));
+    }
+    WR(ybar, 27 * C * accsz(dtype));
+    std::memset(z0, 0, accsz(dtype));
   });
   return 0;
 }

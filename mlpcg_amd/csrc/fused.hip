@@ -23,6 +23,8 @@
 // where 8-plane tiles give too few blocks); x and img of the tile plus a one-cell halo are
 // staged in LDS; a wave processes 64 z-voxels of one (plane, row) per step: 4 groups of
 // 16 voxels.
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -31,7 +33,12 @@ namespace mlpcg {
 namespace {
 
 constexpr int kFz = 64, kFj = 8;  // tile: FI planes (8, or 4 / 2 on small grids) x 8 rows x 64 z
-constexpr int kPz = kFz + 2, kPj = kFj + 2;
+constexpr int kRowCells = kFz + 2, kPj = kFj + 2;  // staged cells per halo row, rows per plane
+// LDS row stride (elements): 40 words, so the plane stride is 400 words.  With 32 banks of
+// one word, column (d1, d2) of the halo tile then starts at bank 16 d1 + 8 d2 (mod 32),
+// which the column-to-lane-group assignment below turns into conflict-free reads.
+constexpr int kPz = 80;
+static_assert(kPz >= kRowCells && (kPz / 2) % 32 == 8 && (kPj * kPz / 2) % 32 == 16, "bank layout");
 __host__ __device__ constexpr int tile_cells(int FI) { return (FI + 2) * kPj * kPz; }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -57,32 +64,40 @@ __host__ __device__ constexpr int nkc(int C) { return C; }
 // addresses for every lane, so a lane's eight K slots come from four 32-bit reads.
 // Slot (g, j) of a 32-slot chunk (lane group g = lane / 16 holds slots 8g..8g+7 of the
 // MFMA operands) carries tap fused_tap(par, g, j) (q = 9(d1+1) + 3(d2+1) + (d3+1) =
-// 3c + d3 + 1 for the (d1, d2) column c):
-//   even z: j = 0..5: (c1,-1) (c1,0) (c1,+1) (c2,+1) (c2,-1) (c2,0); j = 6, 7: (8,-1) (8,0)
-//           for g = 0, (8,+1) (spare) for g = 1;
-//   odd z:  j = 0..5: (c1,0) (c1,+1) (c1,-1) (c2,-1) (c2,0) (c2,+1); j = 6, 7: (8,0) (8,+1)
-//           for g = 0, (spare) (8,-1) for g = 1;
-// lane groups 2 and 3 have spares in j = 6, 7.  Registers hold slot pairs (2i, 2i+1): the
-// pairs (c1,-1)(c1,0), (c2,-1)(c2,0) of an even voxel and (c1,0)(c1,+1), (c2,0)(c2,+1) of
-// an odd voxel are aligned words, and the remaining pair is put together from the halves
-// of two words that the other parity reads as its pairs.  Spare slots read valid cells
-// and get zero weights.  Columns go to lane groups as (c1, c2) = (0, 1), (3, 5), (4, 2),
-// (6, 7).  One chunk per image channel; the same table orders the rows of D (the taps of
-// K), so the x patches of the second product use the same reads.
-__host__ __device__ constexpr int fused_col(int i) { return i == 2 ? 3 : (i == 3 ? 5 : (i == 5 ? 2 : i)); }
+// 3c + d3 + 1 for the (d1, d2) column c, s = 7 the column split over lane groups 0, 2):
+//   even z: j = 0..5: (c1,-1) (c1,0) (c1,+1) (c2,+1) (c2,-1) (c2,0); j = 6, 7: (s,-1) (s,0)
+//           for g = 0, (s,+1) (spare) for g = 2;
+//   odd z:  j = 0..5: (c1,0) (c1,+1) (c1,-1) (c2,-1) (c2,0) (c2,+1); j = 6, 7: (s,0) (s,+1)
+//           for g = 0, (spare) (s,-1) for g = 2;
+// lane groups 1 and 3 have spares in j = 6, 7 (group 1's carry the bias).  Registers hold
+// slot pairs (2i, 2i+1): the pairs (c1,-1)(c1,0), (c2,-1)(c2,0) of an even voxel and
+// (c1,0)(c1,+1), (c2,0)(c2,+1) of an odd voxel are aligned words, and the remaining pair is
+// put together from the halves of two words that the other parity reads as its pairs.
+// Spare slots read valid cells and get zero weights.  Columns go to lane groups as
+// (c1, c2) = (4, 0), (1, 2), (5, 6), (3, 8): the LDS serves a wave's read in two halves
+// of 32 lanes (lane groups 0, 1 and 2, 3), and in the padded tile (column c at bank
+// 16 d1 + 8 d2 mod 32) the two groups of a half always read columns 16 banks apart, each
+// group 16 consecutive words: every read is free of bank conflicts.  Lane groups 1 and 3
+// read their spare s-column words at the addresses of groups 0 and 2 (a broadcast).  One
+// chunk per image channel; the same table orders the rows of D (the taps of K), so the x
+// patches of the second product use the same reads.
+constexpr int kSplitCol = 7;
+__host__ __device__ constexpr int fused_col(int i) {
+  return i == 0 ? 4 : i == 1 ? 0 : i == 2 ? 1 : i == 3 ? 2 : i == 4 ? 5 : i == 5 ? 6 : i == 6 ? 3 : 8;
+}
 __host__ __device__ constexpr int fused_tap(int par, int g, int j) {
-  const int c1 = fused_col(2 * g), c2 = fused_col(2 * g + 1);
+  const int c1 = fused_col(2 * g), c2 = fused_col(2 * g + 1), s = 3 * kSplitCol;
   if (par == 0) {
     return j == 0 ? 3 * c1 : j == 1 ? 3 * c1 + 1 : j == 2 ? 3 * c1 + 2 : j == 3 ? 3 * c2 + 2 :
-           j == 4 ? 3 * c2 : j == 5 ? 3 * c2 + 1 : j == 6 ? (g == 1 ? 26 : 24) : 25;
+           j == 4 ? 3 * c2 : j == 5 ? 3 * c2 + 1 : j == 6 ? (g == 2 ? s + 2 : s) : s + 1;
   }
   return j == 0 ? 3 * c1 + 1 : j == 1 ? 3 * c1 + 2 : j == 2 ? 3 * c1 : j == 3 ? 3 * c2 :
-         j == 4 ? 3 * c2 + 1 : j == 5 ? 3 * c2 + 2 : j == 6 ? 25 : (g == 1 ? 24 : 26);
+         j == 4 ? 3 * c2 + 1 : j == 5 ? 3 * c2 + 2 : j == 6 ? s + 1 : (g == 2 ? s : s + 2);
 }
-constexpr int kBiasGroup = 2;             // lane group whose spare slots carry the bias
+constexpr int kBiasGroup = 1;             // lane group whose spare slots carry the bias
 constexpr uint32_t kOnesF16 = 0x3c003c00u;  // two FP16 ones
 __host__ __device__ constexpr bool fused_slot_valid(int par, int g, int j) {
-  return j < 6 || g == 0 || (g == 1 && j == (par == 0 ? 6 : 7));
+  return j < 6 || g == 0 || (g == 2 && j == (par == 0 ? 6 : 7));
 }
 // D row rho is slot (g, j) of the second product's K: rho = 4g + j (j < 4), 16 + 4g + j - 4.
 __host__ __device__ constexpr int row_g(int rho) { return (rho & 15) >> 2; }
@@ -109,21 +124,17 @@ __device__ __forceinline__ void lane_store(T* base, unsigned boff, T v) {
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* p, int bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
 }
-__device__ __forceinline__ bf16 rsrc_load(__amdgpu_buffer_rsrc_t r, int boff, const float*) {
-  return (bf16)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+// element at lane byte offset boff (range-checked) plus the uniform byte offset soff
+__device__ __forceinline__ bf16 rsrc_load(__amdgpu_buffer_rsrc_t r, unsigned boff, int soff, const float*) {
+  return (bf16)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, boff, soff, 0));
 }
-__device__ __forceinline__ bf16 rsrc_load(__amdgpu_buffer_rsrc_t r, int boff, const bf16*) {
-  return __builtin_bit_cast(bf16, __builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0));
+__device__ __forceinline__ bf16 rsrc_load(__amdgpu_buffer_rsrc_t r, unsigned boff, int soff, const bf16*) {
+  return __builtin_bit_cast(bf16, __builtin_amdgcn_raw_buffer_load_b16(r, boff, soff, 0));
 }
-__device__ __forceinline__ f16 rsrc_load(__amdgpu_buffer_rsrc_t r, int boff, const f16*) {
-  return __builtin_bit_cast(f16, __builtin_amdgcn_raw_buffer_load_b16(r, boff, 0, 0));
+__device__ __forceinline__ f16 rsrc_load(__amdgpu_buffer_rsrc_t r, unsigned boff, int soff, const f16*) {
+  return __builtin_bit_cast(f16, __builtin_amdgcn_raw_buffer_load_b16(r, boff, soff, 0));
 }
 
-// 32-bit LDS word at an even element index (both 16-bit halves)
-template <typename T>
-__device__ __forceinline__ uint32_t lds_word(const T* base, int elem) {
-  return *(const uint32_t*)(base + elem);
-}
 // low halves of a and b (a in the low half of the result); high halves likewise
 __device__ __forceinline__ uint32_t pack_lo(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
 __device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x07060302u); }
@@ -175,8 +186,15 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
 // solver's dot products) run without the others' dead loads, selects and FP64 FMAs.
 constexpr int kEpiPrev = 1, kEpiMaskA = 2, kEpiMaskB = 4, kEpiDot0 = 8, kEpiDot1 = 16;
 constexpr int kEpiRuntime = -1;
+// Occupancy target: the 8-plane LDS tiles of one channel allow 5 workgroups (20 waves) per
+// CU; one-channel kernels without the FP64 dot epilogue fit that in 96 VGPRs (the unrolled
+// loop would otherwise hoist LDS reads into more registers), the others keep what they need.
+__host__ __device__ constexpr int fused_waves(int C, int EPI) {
+  return (C == 1 && EPI >= 0 && !(EPI & (kEpiDot0 | kEpiDot1))) ? 5 : 1;
+}
 template <typename TX, typename TY, typename TP, int C, int FI, int EPI>
-__global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(fused_waves(C, EPI), 8)))
+void sm_fused_kernel(FusedArgs a) {
   constexpr int kTile = tile_cells(FI);
   constexpr int kRows = (FI + 2) * kPj / 4;  // halo-tile rows staged per wave
   typedef Mfma<bf16>::v8 v8;
@@ -205,80 +223,128 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
 #pragma unroll
       for (int kc = 0; kc < NK; ++kc)
         A[par][h][kc] = *(const h8*)(apack + ((((long)par * 2 + h) * NK + kc) * 64 + lane) * 8);
-  const bool bias_lane = g == kBiasGroup;  // its column-8 word of channel 0 carries (1, 1)
+  const bool bias_lane = g == kBiasGroup;  // its split-column word of channel 0 carries (1, 1)
   const float scale = a.in_scale ? (float)(*a.in_scale) : 1.f;
   const float c0 = a.prev ? *(const float*)a.c0 : 0.f;
   const float c1 = a.prev ? *(const float*)a.c1 : 0.f;
 
-  // stage x and img of the halo tile (BF16); zero outside the grid.  Each row of the tile
-  // is read through a buffer resource over its in-grid part (none for rows outside the
-  // grid): the range check returns 0 for the halo cells, so there are no clamps or
-  // selects, and the lane offsets are the same for every row.  Rows r = w + 4 k
-  // (k < kRows) of the (FI + 2) x 10 rows; element e = lane of each row, and e = 64 + lane
-  // for lanes 0, 1.
+  // stage x and img of the halo tile (BF16 x, FP16 img); zero outside the grid.  Plane pi
+  // of the tile (grid plane gi = i0 + pi - 1) is read through a buffer resource over that
+  // grid plane, a row of it through the scalar offset and a cell through the lane offset,
+  // which is out of range for cells outside [0, n3): the range check turns every halo cell
+  // into a zero, with no clamps or selects.  Planes and rows outside the grid get an empty
+  // resource.  Wave w stages rows r = kRows w + k (k < kRows) of the (FI + 2) x 10 rows
+  // (three or four planes, so few resources), element e = lane of each row and e = 64 +
+  // lane for lanes 0, 1.  The staging code is instantiated once per wave index (a uniform
+  // branch), so the row geometry is folded at compile time and a row costs a few scalar
+  // instructions.
   const TX* __restrict__ x = (const TX*)a.x;
   const f16* __restrict__ img = (const f16*)a.img;
-  static_assert((FI + 2) * kPj % 4 == 0 && kPz == 66, "staging assumes 4 waves x kRows rows of 66 cells");
-  int zb[2];  // byte offsets of the lane's cells in a row (negative wraps: out of range)
+  static_assert((FI + 2) * kPj % 4 == 0 && kRowCells == 66, "staging assumes 4 waves x kRows rows of 66 cells");
+  {
+    constexpr unsigned kOut = 0x80000000u;  // beyond any plane (planes are under 2^31 bytes)
+    unsigned zx[2], zi[2];                  // lane byte offsets of its cells in a row
 #pragma unroll
-  for (int u = 0; u < 2; ++u) zb[u] = (z0 - 1 + (u == 0 ? lane : 64 + (lane & 1))) * (int)sizeof(TX);
+    for (int u = 0; u < 2; ++u) {
+      const int z = z0 - 1 + (u == 0 ? lane : 64 + (lane & 1));
+      const bool zok = z >= 0 && z < a.n3;
+      zx[u] = zok ? (unsigned)z * (unsigned)sizeof(TX) : kOut;
+      zi[u] = zok ? (unsigned)z * 2u : kOut;
+    }
+    const long plane = (long)a.n2 * a.n3;
+    const int xpb = (int)(plane * (long)sizeof(TX)), ipb = (int)(plane * 2);  // plane bytes
+    const int xrb = a.n3 * (int)sizeof(TX), irb = a.n3 * 2;                   // row bytes
+    const bool lo_i = i0 > 0, hi_i = i0 + FI < a.n1, lo_j = j0 > 0, hi_j = j0 + kFj < a.n2;
+    const long p0 = (long)(i0 - 1) * plane;  // grid plane of tile plane 0 (-1 on the first tile)
+    auto stage = [&](auto wc) {
+      constexpr int W = decltype(wc)::value;
 #pragma unroll
-  for (int k0 = 0; k0 < kRows; k0 += 13) {
-    constexpr int KB = 13;
-    bf16 xv[KB][2];
-    f16 iv[KB][C][2];
+      for (int k0 = 0; k0 < kRows; k0 += 13) {
+        constexpr int KB = 13;
+        bf16 xv[KB][2];
+        f16 iv[KB][C][2];
 #pragma unroll
-    for (int kk = 0; kk < KB; ++kk) {
-      const int r = w + 4 * (k0 + kk);
-      if (k0 + kk < kRows) {
-        const int pi = r / kPj, pj = r - pi * kPj;
-        const int gi = i0 + pi - 1, gj = j0 + pj - 1;
-        const bool rok = gi >= 0 && gi < a.n1 && gj >= 0 && gj < a.n2;
-        const long rb = ((long)min(max(gi, 0), a.n1 - 1) * a.n2 + min(max(gj, 0), a.n2 - 1)) * a.n3;
-        const int xbytes = rok ? a.n3 * (int)sizeof(TX) : 0;
-        const auto xr = row_rsrc(x + rb, xbytes);
+        for (int kk = 0; kk < KB; ++kk) {
+          const int k = k0 + kk;
+          if (k < kRows) {
+            const int r = kRows * W + k, pi = r / kPj, pj = r % kPj;
+            // boundary planes and rows are tested at run time, the others are in the grid
+            const bool edge_p = pi == 0 || pi == FI + 1, edge_j = pj == 0 || pj == kPj - 1;
+            const bool pok = !edge_p || (pi == 0 ? lo_i : hi_i);
+            const bool rok = pok && (!edge_j || (pj == 0 ? lo_j : hi_j));
+            const long po = edge_p ? (pok ? (long)pi * plane : plane) : (long)pi * plane;
+            const int soff = (j0 + pj - 1);
+            const auto xr = row_rsrc(x + (p0 + po), rok ? xpb : 0);
 #pragma unroll
-        for (int u = 0; u < 2; ++u) xv[kk][u] = rsrc_load(xr, zb[u], (const TX*)nullptr);
+            for (int u = 0; u < 2; ++u) xv[kk][u] = rsrc_load(xr, zx[u], soff * xrb, (const TX*)nullptr);
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const auto ir = row_rsrc(img + c * V + rb, rok ? a.n3 * 2 : 0);
+            for (int c = 0; c < C; ++c) {
+              const auto ir = row_rsrc(img + (c * V + p0 + po), rok ? ipb : 0);
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const int zh = sizeof(TX) == 2 ? zb[u] : zb[u] / 2;
-            iv[kk][c][u] = rsrc_load(ir, zh, (const f16*)nullptr);
+              for (int u = 0; u < 2; ++u) iv[kk][c][u] = rsrc_load(ir, zi[u], soff * irb, (const f16*)nullptr);
+            }
+          }
+        }
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk) {
+          const int r = kRows * W + k0 + kk;
+          if (k0 + kk < kRows) {
+            xs[r * kPz + lane] = xv[kk][0];
+#pragma unroll
+            for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + lane] = iv[kk][c][0];
+          }
+        }
+        if (lane < 2) {
+#pragma unroll
+          for (int kk = 0; kk < KB; ++kk) {
+            const int r = kRows * W + k0 + kk;
+            if (k0 + kk < kRows) {
+              xs[r * kPz + 64 + lane] = xv[kk][1];
+#pragma unroll
+              for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + 64 + lane] = iv[kk][c][1];
+            }
           }
         }
       }
-    }
-#pragma unroll
-    for (int kk = 0; kk < KB; ++kk) {
-      const int r = w + 4 * (k0 + kk);
-      if (k0 + kk < kRows) {
-        xs[r * kPz + lane] = xv[kk][0];
-#pragma unroll
-        for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + lane] = iv[kk][c][0];
-        if (lane < 2) {
-          xs[r * kPz + 64 + lane] = xv[kk][1];
-#pragma unroll
-          for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + 64 + lane] = iv[kk][c][1];
-        }
-      }
+    };
+    switch (w) {
+      case 0: stage(std::integral_constant<int, 0>()); break;
+      case 1: stage(std::integral_constant<int, 1>()); break;
+      case 2: stage(std::integral_constant<int, 2>()); break;
+      default: stage(std::integral_constant<int, 3>()); break;
     }
   }
 
   // per-lane LDS word offsets (even element indices relative to the chunk's cell cb + 32h):
   // image words of this lane's voxel 2nn (even group; the odd group reads the same words),
   // x words of voxel sigma_t(nn) of group t
-  const int o1 = col_off(fused_col(2 * g)) - 1, o2 = col_off(fused_col(2 * g + 1)) - 1, o8 = col_off(8) - 1;
-  const int r3e = g == 1 ? 2 : 0, r3o = g == 1 ? 0 : 2;  // column-8 word: even / odd groups
+  const int o1 = col_off(fused_col(2 * g)) - 1, o2 = col_off(fused_col(2 * g + 1)) - 1;
+  const int o8 = col_off(kSplitCol) - 1;
+  // split-column word of the even / odd groups (groups 1, 3 read those of groups 0, 2)
+  const int r3e = g >= 2 ? 2 : 0, r3o = g >= 2 ? 0 : 2;
   const int iP = o1 + 2 * nn, iQ = o2 + 2 * nn, i8e = o8 + 2 * nn + r3e, i8o = o8 + 2 * nn + r3o;
-  int xP[4], xQ[4], x8[4];
+  // One byte-address register per word (with the wave's row offset folded in): the chunk
+  // loop is unrolled, so each read is its own register plus a compile-time offset.  With
+  // distinct registers no two reads can be merged into a ds_read2, whose 8-bit offsets
+  // would need a fresh base (a VALU add) per chunk, and whose register pairs would need
+  // moves into the MFMA operand tuples.
+  const int wrow = w * kPz;
+  auto lds_base = [&](int elem) {
+    unsigned b = (unsigned)(elem + wrow) * 2u;
+    asm volatile("" : "+v"(b));
+    return b;
+  };
+  unsigned bI[6] = {lds_base(iP), lds_base(iP + 2), lds_base(iQ), lds_base(iQ + 2), lds_base(i8e), lds_base(i8o)};
+  unsigned bX[4][5];
 #pragma unroll
   for (int tt = 0; tt < 4; ++tt) {
     const int sg = 4 * (((nn >> 2) - tt) & 3) + (nn & 3);
-    xP[tt] = o1 + 2 * sg;
-    xQ[tt] = o2 + 2 * sg;
-    x8[tt] = o8 + 2 * sg + ((tt & 1) ? r3o : r3e);
+    const int xP = o1 + 2 * sg, xQ = o2 + 2 * sg, x8 = o8 + 2 * sg + ((tt & 1) ? r3o : r3e);
+    bX[tt][0] = lds_base(xP);
+    bX[tt][1] = lds_base(xP + 2);
+    bX[tt][2] = lds_base(xQ);
+    bX[tt][3] = lds_base(xQ + 2);
+    bX[tt][4] = lds_base(x8);
   }
   const int tstar = (g - (nn >> 2)) & 3;
   const bool odd_e = (nn & 1) != 0, high_e = (nn & 2) != 0;  // element nn % 4 of the result
@@ -312,33 +378,27 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
     return ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0;
   };
   // the LDS words one half-chunk (h) of chunk cell cb needs: per image channel p0, p2, q0,
-  // q2, the column-8 words of the even and odd groups; the x words of the two groups
+  // q2, the split-column words of the even and odd groups; the x words of the two groups
   struct Words {
     uint32_t im[NK][6];
     uint32_t xw[2][5];
   };
-  auto cb_of = [&](int q) { return (((q >> 1) + 1) * kPj + (w + 4 * (q & 1) + 1)) * kPz + 1; };
+  // chunk q's cell cb, less the wave's row offset (a compile-time constant in the unrolled loop)
+  auto cb_of = [&](int q) { return (((q >> 1) + 1) * kPj + (4 * (q & 1) + 1)) * kPz + 1; };
   auto load_words = [&](int cbq, int h) {
     Words r;
+    const int cbyte = 2 * (cbq + 32 * h);
 #pragma unroll
     for (int kc = 0; kc < NK; ++kc) {
-      const f16* ib = is + kc * kTile + cbq + 32 * h;
-      r.im[kc][0] = lds_word(ib, iP);
-      r.im[kc][1] = lds_word(ib, iP + 2);
-      r.im[kc][2] = lds_word(ib, iQ);
-      r.im[kc][3] = lds_word(ib, iQ + 2);
-      r.im[kc][4] = lds_word(ib, i8e);
-      r.im[kc][5] = lds_word(ib, i8o);
+      const char* ib = (const char*)(is + kc * kTile) + cbyte;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) r.im[kc][k] = *(const uint32_t*)(ib + bI[k]);
     }
-    const bf16* xb = xs + cbq + 32 * h;
+    const char* xb = (const char*)xs + cbyte;
 #pragma unroll
     for (int par = 0; par < 2; ++par) {
-      const int tt = 2 * h + par;
-      r.xw[par][0] = lds_word(xb, xP[tt]);
-      r.xw[par][1] = lds_word(xb, xP[tt] + 2);
-      r.xw[par][2] = lds_word(xb, xQ[tt]);
-      r.xw[par][3] = lds_word(xb, xQ[tt] + 2);
-      r.xw[par][4] = lds_word(xb, x8[tt]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) r.xw[par][k] = *(const uint32_t*)(xb + bX[2 * h + par][k]);
     }
     return r;
   };
@@ -348,12 +408,8 @@ __global__ __launch_bounds__(kThreads) void sm_fused_kernel(FusedArgs a) {
   TP pv = LP ? lane_at(pP + vr, mop) : TP(0.f);
   float da = (LD0 && dots) ? lane_at(d0p + vr, mof) : 0.f;
   float db = (LD1 && dots) ? lane_at(d1p + vr * rstep1, d1off) : 0.f;
-  // two chunks per trip: the odd chunk's LDS words sit 4 tile rows (528 bytes) past the
-  // even chunk's, within the read2 immediate range, so both share the lane addresses
-#pragma unroll 2
+#pragma unroll
   for (int q = 0; q < FI * 2; ++q) {
-    const int ii = q >> 1, jj = w + 4 * (q & 1);
-    const int cb = ((ii + 1) * kPj + (jj + 1)) * kPz + 1;
     // next chunk's epilogue operands
     const long vn = row(q + 1 < FI * 2 ? q + 1 : q);
     const uint8_t ka_n = LA ? lane_at(mA + vn, mob) : uint8_t(1);
@@ -441,7 +497,9 @@ unsigned fused_blocks(int n1, int n2, int n3) {
 }
 
 bool fused_eligible(int C, int B, int n1, int n2, int n3) {
-  return (C == 1 || C == 2) && B == 1 && n1 % 8 == 0 && n2 % kFj == 0 && n3 % kFz == 0 && n3 < (1 << 24);
+  // a grid plane of FP32 x is addressed with 32-bit buffer offsets (under 2^31 bytes)
+  return (C == 1 || C == 2) && B == 1 && n1 % 8 == 0 && n2 % kFj == 0 && n3 % kFz == 0 &&
+         (long)n2 * n3 * 4 < (1L << 31);
 }
 
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {

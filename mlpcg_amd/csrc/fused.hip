@@ -3,21 +3,22 @@
 //   y(v) = sum_p K_p(v) x(v + d_p),   K_p(v) = b_p + sum_{c,s} W[p,c,s] img[c, v + d_s]
 //
 // (sm_block_3d_kernel.cu:46-70).  Instead of streaming a stored K (54 of the 60 bytes per
-// voxel of the streamed BF16 apply), both contractions run on the MFMA units, 16 voxels
-// at a time:
+// voxel of the streamed BF16 apply), K is recomputed on the MFMA units, 16 voxels at a
+// time, and applied where the product leaves it:
 //   1. D[p][v] = b_p + sum_k W[p][k] P[k][v]: W is 27 taps x 27C inputs padded to
-//      32 x 32 NKC, P the image patches of the 16 voxels (an im2col read from LDS);
-//   2. y(v) = sum_p x(v + d_p) D[p][v] as the diagonal of a second 16 x 16 x 32 product
-//      of the x patches with D, which is already in the right lanes after step 1.
+//      32 x 32 NKC, P the image patches of the 16 voxels (an im2col read from LDS), the
+//      bias the accumulator;
+//   2. y(v) = sum_p x(v + d_p) D[p][v]: every lane holds 8 taps of one voxel after step 1
+//      and dots them with the matching x words (BF16 dot2); a cross-lane reduction over
+//      the four lane groups completes the sums.
 // HBM traffic is x, img and y (+ epilogue operands): about 8 B/voxel instead of 60.
 //
-// Operand layout of v_mfma_f32_16x16x32_{f16,bf16} (lane l, element j / i):
+// Operand layout of v_mfma_f32_16x16x32_f16 (lane l, element j / i):
 //   A[l % 16][8 (l / 16) + j],  B[8 (l / 16) + j][l % 16],  D[4 (l / 16) + i][l % 16].
 // Precision: step 1 takes img and W in FP16 (11-bit significands: image values lie in
-// [0, 1] and weights are O(1), so the range is ample) with FP32 accumulation, the bias
-// entering as an FP16 value plus its FP16 remainder on two spare K slots (about 22
-// significant bits), which makes K more accurate than a FP32-computed K rounded to BF16;
-// step 2 rounds x and K to BF16 (the storage type of the plan) and accumulates in FP32.
+// [0, 1] and weights are O(1), so the range is ample) with FP32 accumulation and an FP32
+// bias, which makes K more accurate than a FP32-computed K rounded to BF16; step 2 rounds
+// x and K to BF16 (the storage type of the plan) and accumulates in FP32.
 //
 // Tile: a block of 4 waves covers FI planes x 8 rows x 64 z-voxels (FI = 8, or 4 / 2
 // where 8-plane tiles give too few blocks); x and img of the tile plus a one-cell halo are
@@ -42,14 +43,9 @@ static_assert(kPz >= kRowCells && (kPz / 2) % 32 == 8 && (kPj * kPz / 2) % 32 ==
 __host__ __device__ constexpr int tile_cells(int FI) { return (FI + 2) * kPj * kPz; }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 template <typename TW> struct Mfma;
-template <> struct Mfma<bf16> {
-  typedef bf16 v8 __attribute__((ext_vector_type(8)));
-  static __device__ __forceinline__ f32x4 run(v8 a, v8 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-  }
-};
 template <> struct Mfma<f16> {
   typedef f16 v8 __attribute__((ext_vector_type(8)));
   static __device__ __forceinline__ f32x4 run(v8 a, v8 b, f32x4 c) {
@@ -69,7 +65,7 @@ __host__ __device__ constexpr int nkc(int C) { return C; }
 //           for g = 0, (s,+1) (spare) for g = 2;
 //   odd z:  j = 0..5: (c1,0) (c1,+1) (c1,-1) (c2,-1) (c2,0) (c2,+1); j = 6, 7: (s,0) (s,+1)
 //           for g = 0, (spare) (s,-1) for g = 2;
-// lane groups 1 and 3 have spares in j = 6, 7 (group 1's carry the bias).  Registers hold
+// lane groups 1 and 3 have spares in j = 6, 7.  Registers hold
 // slot pairs (2i, 2i+1): the pairs (c1,-1)(c1,0), (c2,-1)(c2,0) of an even voxel and
 // (c1,0)(c1,+1), (c2,0)(c2,+1) of an odd voxel are aligned words, and the remaining pair is
 // put together from the halves of two words that the other parity reads as its pairs.
@@ -94,8 +90,6 @@ __host__ __device__ constexpr int fused_tap(int par, int g, int j) {
   return j == 0 ? 3 * c1 + 1 : j == 1 ? 3 * c1 + 2 : j == 2 ? 3 * c1 : j == 3 ? 3 * c2 :
          j == 4 ? 3 * c2 + 1 : j == 5 ? 3 * c2 + 2 : j == 6 ? s + 1 : (g == 2 ? s : s + 2);
 }
-constexpr int kBiasGroup = 1;             // lane group whose spare slots carry the bias
-constexpr uint32_t kOnesF16 = 0x3c003c00u;  // two FP16 ones
 __host__ __device__ constexpr bool fused_slot_valid(int par, int g, int j) {
   return j < 6 || g == 0 || (g == 2 && j == (par == 0 ? 6 : 7));
 }
@@ -135,6 +129,32 @@ __device__ __forceinline__ f16 rsrc_load(__amdgpu_buffer_rsrc_t r, unsigned boff
   return __builtin_bit_cast(f16, __builtin_amdgcn_raw_buffer_load_b16(r, boff, soff, 0));
 }
 
+// a + b after v_permlane32_swap (lanes 32..63 of a traded with lanes 0..31 of b) or
+// v_permlane16_swap (odd 16-lane rows of a traded with even rows of b).  The ROCm 7.2
+// compiler reads the builtin's second result from the first result's register (a + a)
+// unless both results pass through an empty asm, which pins them to their own registers.
+__device__ __forceinline__ float swap32_sum(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                  false, false);
+  unsigned u = r[0], v = r[1];
+  asm volatile("" : "+v"(u), "+v"(v));
+  return __builtin_bit_cast(float, u) + __builtin_bit_cast(float, v);
+}
+__device__ __forceinline__ float swap16_sum(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                  false, false);
+  unsigned u = r[0], v = r[1];
+  asm volatile("" : "+v"(u), "+v"(v));
+  return __builtin_bit_cast(float, u) + __builtin_bit_cast(float, v);
+}
+// a.lo b.lo + a.hi b.hi of two BF16 pairs in FP32: the first dot of a chain, in the
+// three-operand form with a zero accumulator (for a zero the compiler emits the
+// accumulate-in-place form after a register initialisation)
+__device__ __forceinline__ float dot2_bf16_first(uint32_t a, uint32_t b) {
+  float d;
+  asm("v_dot2_f32_bf16 %0, %1, %2, 0" : "=v"(d) : "v"(a), "v"(b));
+  return d;
+}
 // low halves of a and b (a in the low half of the result); high halves likewise
 __device__ __forceinline__ uint32_t pack_lo(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
 __device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x07060302u); }
@@ -142,43 +162,53 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 }  // namespace
 
-size_t fused_weights_bytes(int C) { return (size_t)2 * 2 * nkc(C) * 64 * 8 * sizeof(f16); }
+// Two packings of the weights (FP16), then the bias accumulators (FP32).
+constexpr size_t fused_apack_bytes(int C) { return (size_t)2 * 2 * nkc(C) * 64 * 8 * sizeof(f16); }
+size_t fused_weights_bytes(int C) { return 2 * fused_apack_bytes(C) + (size_t)2 * 2 * 64 * 4 * sizeof(float); }
+constexpr int kBiasGroup = 1;               // lane group whose spare slots carry the bias (slot form)
+constexpr uint32_t kOnesF16 = 0x3c003c00u;  // two FP16 ones
 
-// apack[par][h][kc][lane][8] (FP16): A[m][k] = W[tap of D row 16h + m][channel kc][tap of
-// slot k] for z parity par, zero for spare slots.  The bias rides on the spare slots
-// (g, j) = (2, 6), (2, 7) of channel 0, whose image values the kernel sets to 1: their
-// weights are the bias split into an FP16 value and its FP16 remainder, so D gets the
-// bias to about 22 bits with FP32 accumulation.
+// apack[form][par][h][kc][lane][8] (FP16): A[m][k] = W[tap of D row 16h + m][channel kc][tap
+// of slot k] for z parity par, zero for spare slots.  Form 0 leaves the bias out: the
+// kernel starts the accumulation from bpack[par][h][lane][4] (FP32), the bias of the tap of
+// D row 16h + 4 (lane / 16) + i, the element i that lane holds of the result (exact, but
+// 16 registers).  Form 1 puts the bias on the spare slots (1, 6), (1, 7) of channel 0,
+// whose image values the kernel sets to 1, as an FP16 value plus its FP16 remainder (about
+// 22 significant bits), for kernels short of registers.
 __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __restrict__ W,
                                                               const float* __restrict__ b, int C,
                                                               f16* __restrict__ apack) {
   const int NK = nkc(C);
   const int total = 2 * 2 * NK * 64 * 8;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+  for (int e = threadIdx.x; e < 2 * total; e += blockDim.x) {
     const int j = e % 8, lane = (e / 8) % 64, kc = (e / 512) % NK, h = (e / (512 * NK)) % 2;
-    const int par = e / (1024 * NK);
+    const int par = (e / (1024 * NK)) % 2, form = e / total;
     const int rho = 16 * h + lane % 16, gk = lane / 16;
     const bool rok = fused_slot_valid(par, row_g(rho), row_j(rho));
     const int p = fused_tap(par, row_g(rho), row_j(rho)), q = fused_tap(par, gk, j);
-    float v = 0.f;
-    if (rok && fused_slot_valid(par, gk, j)) {
-      v = W[((long)p * C + kc) * 27 + q];
-    } else if (rok && kc == 0 && gk == kBiasGroup) {
+    float v = rok && fused_slot_valid(par, gk, j) ? W[((long)p * C + kc) * 27 + q] : 0.f;
+    if (form == 1 && rok && kc == 0 && gk == kBiasGroup && j >= 6) {
       const float hi = (float)(f16)b[p];
-      v = j == 6 ? hi : (j == 7 ? b[p] - hi : 0.f);
+      v = j == 6 ? hi : b[p] - hi;
     }
     apack[e] = (f16)v;
   }
+  float* bpack = (float*)((char*)apack + 2 * fused_apack_bytes(C));
+  for (int e = threadIdx.x; e < 2 * 2 * 64 * 4; e += blockDim.x) {
+    const int i = e % 4, lane = (e / 4) % 64, h = (e / 256) % 2, par = e / 512;
+    const int rho = 16 * h + 4 * (lane / 16) + i;
+    const bool rok = fused_slot_valid(par, row_g(rho), row_j(rho));
+    bpack[e] = rok ? b[fused_tap(par, row_g(rho), row_j(rho))] : 0.f;
+  }
 }
 
-// The apply is a second MFMA: after the first, lane (g, n) (g = l / 16, n = l % 16) holds
-// D = K for the taps of rows tau(g, j) = 4g + j (j < 4) and 16 + 4g + j - 4 (j >= 4) of
-// voxel n of the group.  With A2[m][k] = x(voxel sigma(m), tap of row tau(k)) and
-// B2[k][n] = bf16(D[tau(k)][n]), C2[m][n] = sum_k x(sigma(m), .) K(n) is y(n) where
-// sigma(m) = n.  The row permutation sigma_t(m) = 4 ((m/4 - t) mod 4) + m % 4 of group t
-// places y(n) of group t in lane (g, n) with g = (n/4 + t) mod 4, element n % 4: over the
-// 4 groups of a 64-voxel chunk every lane ends up holding exactly one voxel, voxel n of
-// group t* = (g - n/4) mod 4, i.e. z0 + 32 (t*/2) + 2n + t* % 2.
+// The apply: after the first product, lane (g, n) (g = l / 16, n = l % 16) holds D = K for
+// the taps of D rows 4g + i and 16 + 4g + i (i < 4) of voxel n of the group, i.e. of the
+// lane's own eight K slots (g, 0..7).  The x values of those slots are the same LDS words
+// as the image patch, read from the x tile, so each lane forms its share of y(n) with four
+// BF16 dot2 instructions; the four lane groups' shares of the four groups of a 64-voxel
+// chunk are then summed by two cross-lane swaps (32-lane, then 16-lane), which leave y of
+// voxel n of group t = g in lane (g, n): z0 + 32 (g / 2) + 2n + g % 2.
 //
 // EPI selects the epilogue at compile time (kEpiRuntime: every operand tested at run time):
 // each present operand costs its per-chunk load and its VALU work, so the hot launches of
@@ -190,14 +220,13 @@ constexpr int kEpiRuntime = -1;
 // CU; one-channel kernels without the FP64 dot epilogue fit that in 96 VGPRs (the unrolled
 // loop would otherwise hoist LDS reads into more registers), the others keep what they need.
 __host__ __device__ constexpr int fused_waves(int C, int EPI) {
-  return (C == 1 && EPI >= 0 && !(EPI & (kEpiDot0 | kEpiDot1))) ? 5 : 1;
+  return C == 1 && EPI >= 0 ? ((EPI & (kEpiDot0 | kEpiDot1)) ? 4 : 5) : 1;
 }
 template <typename TX, typename TY, typename TP, int C, int FI, int EPI>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(fused_waves(C, EPI), 8)))
 void sm_fused_kernel(FusedArgs a) {
   constexpr int kTile = tile_cells(FI);
   constexpr int kRows = (FI + 2) * kPj / 4;  // halo-tile rows staged per wave
-  typedef Mfma<bf16>::v8 v8;
   typedef Mfma<f16>::v8 h8;
   constexpr int NK = nkc(C);
   __shared__ __attribute__((aligned(16))) bf16 xs[kTile];
@@ -214,7 +243,9 @@ void sm_fused_kernel(FusedArgs a) {
   const long V = (long)a.n1 * a.n2 * a.n3;
 
   // per-lane constant operands (global loads issued before the tile staging)
-  const f16* apack = (const f16*)a.apack;
+  // the bias as the accumulator (kernels without the FP64 dot epilogue) or on spare slots
+  constexpr bool BA = EPI >= 0 && !(EPI & (kEpiDot0 | kEpiDot1));
+  const f16* apack = (const f16*)((const char*)a.apack + (BA ? 0 : fused_apack_bytes(C)));
   h8 A[2][2][NK];  // [z parity][row half][channel]
 #pragma unroll
   for (int par = 0; par < 2; ++par)
@@ -223,7 +254,14 @@ void sm_fused_kernel(FusedArgs a) {
 #pragma unroll
       for (int kc = 0; kc < NK; ++kc)
         A[par][h][kc] = *(const h8*)(apack + ((((long)par * 2 + h) * NK + kc) * 64 + lane) * 8);
-  const bool bias_lane = g == kBiasGroup;  // its split-column word of channel 0 carries (1, 1)
+  const float* bpack = (const float*)((const char*)a.apack + 2 * fused_apack_bytes(C));
+  f32x4 bias[2][2];  // [z parity][row half]
+#pragma unroll
+  for (int par = 0; par < 2; ++par)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      bias[par][h] = BA ? *(const f32x4*)(bpack + ((par * 2 + h) * 64 + lane) * 4) : f32x4{0.f, 0.f, 0.f, 0.f};
+  const bool bias_lane = !BA && g == kBiasGroup;  // its split-column word of channel 0 carries (1, 1)
   const float scale = a.in_scale ? (float)(*a.in_scale) : 1.f;
   const float c0 = a.prev ? *(const float*)a.c0 : 0.f;
   const float c1 = a.prev ? *(const float*)a.c1 : 0.f;
@@ -316,39 +354,28 @@ void sm_fused_kernel(FusedArgs a) {
   }
 
   // per-lane LDS word offsets (even element indices relative to the chunk's cell cb + 32h):
-  // image words of this lane's voxel 2nn (even group; the odd group reads the same words),
-  // x words of voxel sigma_t(nn) of group t
+  // the words of this lane's voxel 2nn (even group; the odd group reads the same words),
+  // at the same offsets in the image and in the x tile
   const int o1 = col_off(fused_col(2 * g)) - 1, o2 = col_off(fused_col(2 * g + 1)) - 1;
   const int o8 = col_off(kSplitCol) - 1;
   // split-column word of the even / odd groups (groups 1, 3 read those of groups 0, 2)
   const int r3e = g >= 2 ? 2 : 0, r3o = g >= 2 ? 0 : 2;
   const int iP = o1 + 2 * nn, iQ = o2 + 2 * nn, i8e = o8 + 2 * nn + r3e, i8o = o8 + 2 * nn + r3o;
   // One byte-address register per word (with the wave's row offset folded in): the chunk
-  // loop is unrolled, so each read is its own register plus a compile-time offset.  With
-  // distinct registers no two reads can be merged into a ds_read2, whose 8-bit offsets
-  // would need a fresh base (a VALU add) per chunk, and whose register pairs would need
-  // moves into the MFMA operand tuples.
+  // loop is unrolled, so each read is its own register plus a compile-time offset (the
+  // tile array and the chunk).  With distinct registers no two reads can be merged into a
+  // ds_read2, whose 8-bit offsets would need a fresh base (a VALU add) per chunk, and
+  // whose register pairs would need moves into the operand tuples.
   const int wrow = w * kPz;
   auto lds_base = [&](int elem) {
     unsigned b = (unsigned)(elem + wrow) * 2u;
     asm volatile("" : "+v"(b));
     return b;
   };
-  unsigned bI[6] = {lds_base(iP), lds_base(iP + 2), lds_base(iQ), lds_base(iQ + 2), lds_base(i8e), lds_base(i8o)};
-  unsigned bX[4][5];
-#pragma unroll
-  for (int tt = 0; tt < 4; ++tt) {
-    const int sg = 4 * (((nn >> 2) - tt) & 3) + (nn & 3);
-    const int xP = o1 + 2 * sg, xQ = o2 + 2 * sg, x8 = o8 + 2 * sg + ((tt & 1) ? r3o : r3e);
-    bX[tt][0] = lds_base(xP);
-    bX[tt][1] = lds_base(xP + 2);
-    bX[tt][2] = lds_base(xQ);
-    bX[tt][3] = lds_base(xQ + 2);
-    bX[tt][4] = lds_base(x8);
-  }
-  const int tstar = (g - (nn >> 2)) & 3;
-  const bool odd_e = (nn & 1) != 0, high_e = (nn & 2) != 0;  // element nn % 4 of the result
-  const int myz = 32 * (tstar >> 1) + 2 * nn + (tstar & 1);
+  const unsigned bI[6] = {lds_base(iP), lds_base(iP + 2), lds_base(iQ), lds_base(iQ + 2), lds_base(i8e),
+                          lds_base(i8o)};
+  // after the cross-lane reduction, lane (g, nn) holds voxel nn of group t = g
+  const int myz = 32 * (g >> 1) + 2 * nn + (g & 1);
   __syncthreads();
 
   // epilogue operands through always-valid pointers (x has at least 2V bytes), so the
@@ -368,20 +395,21 @@ void sm_fused_kernel(FusedArgs a) {
   const bool dots = RT ? a.partials != nullptr : LD0;
   const float* d0p = a.dot0 ? a.dot0 : (const float*)a.x;
   const float* d1p = a.dot1 ? a.dot1 : (const float*)a.x;
-  const long rstep1 = a.dot1 ? 1 : 0;               // without dot1 every lane reads element 0
+  // without dot1 every lane reads element 0 (a compile-time epilogue with dot1 has it)
+  const bool has_d1 = RT ? a.dot1 != nullptr : LD1;
+  const long rstep1 = has_d1 ? 1 : 0;
   const unsigned mob = (unsigned)myz, mop = mob * (unsigned)sizeof(TP), mof = mob * 4u;
-  const unsigned moy = mob * (unsigned)sizeof(TY), d1off = a.dot1 ? mof : 0u;
+  const unsigned moy = mob * (unsigned)sizeof(TY), d1off = has_d1 ? mof : 0u;
   double red[2] = {0.0, 0.0};
-  const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
   auto row = [&](int q) {
     const int ii = q >> 1, jj = w + 4 * (q & 1);
     return ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0;
   };
-  // the LDS words one half-chunk (h) of chunk cell cb needs: per image channel p0, p2, q0,
-  // q2, the split-column words of the even and odd groups; the x words of the two groups
+  // the LDS words one half-chunk (h) of chunk cell cb needs: per image channel and for x,
+  // p0, p2, q0, q2 and the split-column words of the even and odd groups
   struct Words {
     uint32_t im[NK][6];
-    uint32_t xw[2][5];
+    uint32_t xw[6];
   };
   // chunk q's cell cb, less the wave's row offset (a compile-time constant in the unrolled loop)
   auto cb_of = [&](int q) { return (((q >> 1) + 1) * kPj + (4 * (q & 1) + 1)) * kPz + 1; };
@@ -396,10 +424,7 @@ void sm_fused_kernel(FusedArgs a) {
     }
     const char* xb = (const char*)xs + cbyte;
 #pragma unroll
-    for (int par = 0; par < 2; ++par) {
-#pragma unroll
-      for (int k = 0; k < 5; ++k) r.xw[par][k] = *(const uint32_t*)(xb + bX[2 * h + par][k]);
-    }
+    for (int k = 0; k < 6; ++k) r.xw[k] = *(const uint32_t*)(xb + bI[k]);
     return r;
   };
   Words cur = load_words(cb_of(0), 0);
@@ -417,7 +442,7 @@ void sm_fused_kernel(FusedArgs a) {
     const TP pv_n = LP ? lane_at(pP + vn, mop) : TP(0.f);
     const float da_n = (LD0 && dots) ? lane_at(d0p + vn, mof) : 0.f;
     const float db_n = (LD1 && dots) ? lane_at(d1p + vn * rstep1, d1off) : 0.f;
-    f32x4 c2[4];
+    float part[4];  // this lane's share of y(voxel nn of group t): its 8 K slots
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       // LDS words of the next half-chunk ((q, 1) or (q + 1, 0)) are read before this
@@ -434,34 +459,37 @@ void sm_fused_kernel(FusedArgs a) {
         bv[0][kc] = __builtin_bit_cast(h8, u32x4{m[0], pack_lo(m[1], m[3]), m[2], e8});
         bv[1][kc] = __builtin_bit_cast(h8, u32x4{m[1], pack_hi(m[0], m[2]), m[3], d8});
       }
+      const uint32_t* xw = cur.xw;
+      const uint32_t xs_[2][4] = {{xw[0], pack_lo(xw[1], xw[3]), xw[2], xw[4]},
+                                  {xw[1], pack_hi(xw[0], xw[2]), xw[3], xw[5]}};
 #pragma unroll
       for (int par = 0; par < 2; ++par) {
-        const int tt = 2 * h + par;
-        f32x4 d0 = zero4, d1 = zero4;
+        // K of this lane's 8 slots (D rows 4g + i and 16 + 4g + i of voxel nn); the biases
+        // enter as the accumulator or through the ones slots
+        f32x4 d0 = bias[par][0], d1 = bias[par][1];
 #pragma unroll
         for (int kc = 0; kc < NK; ++kc) {
           d0 = Mfma<f16>::run(A[par][0][kc], bv[par][kc], d0);
           d1 = Mfma<f16>::run(A[par][1][kc], bv[par][kc], d1);
         }
-        const uint32_t* xw = cur.xw[par];
-        const v8 ax = par == 0 ? __builtin_bit_cast(v8, u32x4{xw[0], pack_lo(xw[1], xw[3]), xw[2], xw[4]})
-                               : __builtin_bit_cast(v8, u32x4{xw[1], pack_hi(xw[0], xw[2]), xw[3], xw[4]});
-        v8 bk;
+        // slot pairs (0,1) (2,3) of the first half of D, (4,5) (6,7) of the second, in BF16
+        // against the x words of the same slots
+        const float kf[8] = {d0[0], d0[1], d0[2], d0[3], d1[0], d1[1], d1[2], d1[3]};
+        float acc = 0.f;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          bk[j] = (bf16)d0[j];
-          bk[j + 4] = (bf16)d1[j];
+        for (int k = 0; k < 4; ++k) {
+          const bf16x2 kp = {(bf16)kf[2 * k], (bf16)kf[2 * k + 1]};
+          acc = k == 0 ? dot2_bf16_first(xs_[par][k], __builtin_bit_cast(uint32_t, kp))
+                       : __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, xs_[par][k]), kp, acc, false);
         }
-        c2[tt] = Mfma<bf16>::run(ax, bk, zero4);
+        part[2 * h + par] = acc;
       }
       cur = nxt;
     }
-    // this lane's voxel: group tstar, element nn % 4 (selects, no branches)
-    f32x4 cs = c2[0];
-#pragma unroll
-    for (int tt = 1; tt < 4; ++tt) cs = (tstar == tt) ? c2[tt] : cs;
-    const float lo = odd_e ? cs[1] : cs[0], hi = odd_e ? cs[3] : cs[2];
-    const float ymine = high_e ? hi : lo;
+    // sum the four lane groups' shares: afterwards lane (g, nn) holds y of voxel nn of group
+    // t = g.  The 32-lane swap pairs groups g and g + 2 (t = 0, 1 kept in the low half,
+    // t = 2, 3 in the high half), the 16-lane swap then pairs groups g and g ^ 1.
+    const float ymine = swap16_sum(swap32_sum(part[0], part[2]), swap32_sum(part[1], part[3]));
     float val = ymine * scale;
     if (has_a && ka == 0) val = 0.f;
     if (has_p) val = c0 * (float)pv + c1 * val;

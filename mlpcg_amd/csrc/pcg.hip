@@ -70,16 +70,17 @@ __device__ __forceinline__ bool run_at(long cid, int n1, int n2, int n3, Run<VZ>
 
 // Loads u = x + a*p (or just x when p == nullptr) on the 7-point neighbourhood of a run:
 // c[0..VZ+1] is the centre row with one halo cell each side, s[4][VZ] the four side rows
-// (i-1, i+1, j-1, j+1).  Out-of-grid values are 0.  u is rounded to FP32 (the storage
-// precision of x) so a residual computed from it is the residual of the stored iterate.
+// (i-1, i+1, j-1, j+1).  Out-of-grid values are 0.  u is one FP32 fused multiply-add (the
+// storage precision of x, one rounding) so a residual computed from it is the residual of
+// the stored iterate.
 template <typename TA>
-__device__ __forceinline__ TA upd(float x, float p, TA a, bool has_p) {
-  return has_p ? (TA)(float)((TA)x + a * (TA)p) : (TA)x;
+__device__ __forceinline__ TA upd(float x, float p, float a, bool has_p) {
+  return has_p ? (TA)fmaf(a, p, x) : (TA)x;
 }
 
 template <int VZ, typename TA>
 __device__ __forceinline__ void load_star(const float* __restrict__ x, const float* __restrict__ p,
-                                          TA a, const Run<VZ>& r, int n1, int n2, int n3,
+                                          float a, const Run<VZ>& r, int n1, int n2, int n3,
                                           TA (&c)[VZ + 2], TA (&s)[4][VZ]) {
   const long s1 = (long)n2 * n3;
   const int lane = threadIdx.x & 63;
@@ -121,7 +122,7 @@ template <int VZ>
 __global__ __launch_bounds__(kThreads) void resid_flat_kernel(FlatArgs a, int init) {
   SolveState* st = a.st;
   if (!init && st->done) return;
-  const double alpha = init ? 0.0 : st->alpha;
+  const float alpha = init ? 0.f : (float)st->alpha;
   const float* pp = init ? nullptr : a.p;
   double red[2] = {0.0, 0.0};
   const long nch = (long)a.n1 * a.n2 * (a.n3 / VZ);
@@ -261,7 +262,8 @@ __global__ __launch_bounds__(kThreads) void p_spmv_csr_kernel(CsrArgs a) {
 __global__ __launch_bounds__(kThreads) void resid_csr_kernel(CsrArgs a, int init) {
   SolveState* st = a.st;
   if (!init && st->done) return;
-  const double alpha = init ? 0.0 : st->alpha;
+  // the iterate update is the flat kernel's: one FP32 fused multiply-add
+  const float alpha = init ? 0.f : (float)st->alpha;
   double red[2] = {0.0, 0.0};
   for (long row = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; row < a.n;
        row += (long)gridDim.x * kThreads) {
@@ -269,14 +271,13 @@ __global__ __launch_bounds__(kThreads) void resid_csr_kernel(CsrArgs a, int init
     double ax = 0.0;
     for (int e = e0; e < e1; ++e) {
       const int col = a.col[e];
-      const double xc = init ? (double)a.x_old[col]
-                            : (double)(float)((double)a.x_old[col] + alpha * (double)a.p[col]);
+      const double xc = init ? (double)a.x_old[col] : (double)fmaf(alpha, a.p[col], a.x_old[col]);
       ax += (double)a.val[e] * xc;
     }
     const double bv = a.b[row];
     const double res = bv - ax;
     a.r[row] = (float)res;
-    if (!init) a.x_new[row] = (float)((double)a.x_old[row] + alpha * (double)a.p[row]);
+    if (!init) a.x_new[row] = fmaf(alpha, a.p[row], a.x_old[row]);
     red[0] += res * res;
     red[1] += bv * bv;
   }
@@ -380,14 +381,17 @@ struct DirIn {
   const float* q;
   const float* pN;  // null when no direction is kept
   const float* pO;  // null when fewer than two are kept
-  double gN, gO;
+  float gN, gO;
 };
 
+// Q = q - gN N - gO O in FP32 with fused multiply-adds (one rounding each): Q is stored in
+// FP32, and forming it in FP64 at the five rows of a run's neighbourhood cost more VALU
+// work than the memory time of the kernel allows
 __device__ __forceinline__ float dir_at(const DirIn& d, long v) {
-  double u = (double)d.q[v];
-  if (d.pN) u = fma(-d.gN, (double)d.pN[v], u);
-  if (d.pO) u = fma(-d.gO, (double)d.pO[v], u);
-  return (float)u;
+  float u = d.q[v];
+  if (d.pN) u = fmaf(-d.gN, d.pN[v], u);
+  if (d.pO) u = fmaf(-d.gO, d.pO[v], u);
+  return u;
 }
 
 // Q at VZ consecutive voxels; nv receives N there (0 without N)
@@ -399,11 +403,11 @@ __device__ __forceinline__ void dir_vec(const DirIn& d, long v0, float (&out)[VZ
   if (d.pO) load_vec<VZ>(d.pO + v0, ov);
 #pragma unroll
   for (int t = 0; t < VZ; ++t) {
-    double u = (double)qv[t];
-    if (d.pN) u = fma(-d.gN, (double)nv[t], u);
+    float u = qv[t];
+    if (d.pN) u = fmaf(-d.gN, nv[t], u);
     else nv[t] = 0.f;
-    if (d.pO) u = fma(-d.gO, (double)ov[t], u);
-    out[t] = (float)u;
+    if (d.pO) u = fmaf(-d.gO, ov[t], u);
+    out[t] = u;
   }
 }
 
@@ -444,8 +448,8 @@ __device__ __forceinline__ DirIn dir_in(const float* q, const float* pN, const f
   d.q = q;
   d.pN = pN;
   d.pO = pO;
-  d.gN = pN ? st->proj[0] : 0.0;
-  d.gO = pO ? st->proj[1] : 0.0;
+  d.gN = pN ? (float)st->proj[0] : 0.f;
+  d.gO = pO ? (float)st->proj[1] : 0.f;
   return d;
 }
 

@@ -218,12 +218,18 @@ constexpr int kEpiPrev = 1, kEpiMaskA = 2, kEpiMaskB = 4, kEpiDot0 = 8, kEpiDot1
 constexpr int kEpiRuntime = -1;
 // Occupancy target: the 8-plane LDS tiles of one channel allow 5 workgroups (20 waves) per
 // CU; one-channel kernels without the FP64 dot epilogue fit that in 96 VGPRs (the unrolled
-// loop would otherwise hoist LDS reads into more registers), the others keep what they need.
-__host__ __device__ constexpr int fused_waves(int C, int EPI) {
-  return C == 1 && EPI >= 0 ? ((EPI & (kEpiDot0 | kEpiDot1)) ? 4 : 5) : 1;
+// loop would otherwise hoist LDS reads into more registers), with it in 128.  Three-channel
+// kernels (4-plane tiles, 4 workgroups per CU) fit 128 VGPRs except with two solver dots.
+// 2-plane tiles (small levels) and the others keep what they need (a tighter target made
+// them spill).
+__host__ __device__ constexpr int fused_waves(int C, int FI, int EPI) {
+  return EPI < 0 || FI < 4 ? 1
+         : C == 1 ? ((EPI & (kEpiDot0 | kEpiDot1)) ? 4 : 5)
+         : C == 3 ? ((EPI & kEpiDot1) ? 1 : 4)
+                  : 1;
 }
 template <typename TX, typename TY, typename TP, int C, int FI, int EPI>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(fused_waves(C, EPI), 8)))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(fused_waves(C, FI, EPI), 8)))
 void sm_fused_kernel(FusedArgs a) {
   constexpr int kTile = tile_cells(FI);
   constexpr int kRows = (FI + 2) * kPj / 4;  // halo-tile rows staged per wave
@@ -510,28 +516,32 @@ void sm_fused_kernel(FusedArgs a) {
   if (dots) block_partials<2>(red, a.partials);
 }
 
-// Tile depth along i: the deepest of 8, 4 planes that still gives kMinFusedBlocks
-// workgroups (about eight per CU) to spread over the chip, else 2 (at 128^3: 2048
-// workgroups of 2-plane tiles, 16.0 us, against 1024 of 4-plane tiles, 17.1 us).
+// Tile depth along i: the deepest of 8, 4 planes that keeps the tile's LDS (x and C image
+// channels) within kMaxFusedLds, so at least four workgroups fit a CU, and that still
+// gives kMinFusedBlocks workgroups (about eight per CU) to spread over the chip, else 2 (at
+// 128^3: 2048 workgroups of 2-plane tiles, 16.0 us, against 1024 of 4-plane tiles, 17.1 us).
 constexpr unsigned kMinFusedBlocks = 2048;
+constexpr int kMaxFusedLds = 40 * 1024;
 
-static int fused_fi(int n1, int n2, int n3) {
+static int fused_fi(int C, int n1, int n2, int n3) {
   const unsigned b8 = (unsigned)((n1 / 8) * (n2 / kFj) * (n3 / kFz));
-  return b8 >= kMinFusedBlocks ? 8 : (2 * b8 >= kMinFusedBlocks ? 4 : 2);
+  const int fmax = (C + 1) * tile_cells(8) * 2 <= kMaxFusedLds ? 8 : 4;
+  const int f = b8 >= kMinFusedBlocks ? 8 : (2 * b8 >= kMinFusedBlocks ? 4 : 2);
+  return f < fmax ? f : fmax;
 }
 
-unsigned fused_blocks(int n1, int n2, int n3) {
-  return (unsigned)((n1 / fused_fi(n1, n2, n3)) * (n2 / kFj) * (n3 / kFz));
+unsigned fused_blocks(int C, int n1, int n2, int n3) {
+  return (unsigned)((n1 / fused_fi(C, n1, n2, n3)) * (n2 / kFj) * (n3 / kFz));
 }
 
 bool fused_eligible(int C, int B, int n1, int n2, int n3) {
   // a grid plane of FP32 x is addressed with 32-bit buffer offsets (under 2^31 bytes)
-  return (C == 1 || C == 2) && B == 1 && n1 % 8 == 0 && n2 % kFj == 0 && n3 % kFz == 0 &&
+  return C >= 1 && C <= 3 && B == 1 && n1 % 8 == 0 && n2 % kFj == 0 && n3 % kFz == 0 &&
          (long)n2 * n3 * 4 < (1L << 31);
 }
 
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
-  MLPCG_REQUIRE(C == 1 || C == 2, "fused_pack: C must be 1 or 2");
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "fused_pack: C must be 1, 2 or 3");
   MLPCG_REQUIRE(tw == kBF16, "fused_pack: BF16 plans only (got %d)", tw);
   fused_pack_kernel<<<1, kThreads, 0, st>>>((const float*)W, (const float*)b, C, (f16*)wbuf);
   return check_launch("fused_pack");
@@ -539,10 +549,12 @@ int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, h
 
 template <typename TX, typename TY, int C, int EPI>
 static void launch_fused_t(const FusedArgs& a, hipStream_t st) {
-  const unsigned nb = fused_blocks(a.n1, a.n2, a.n3);
-  const int fi = fused_fi(a.n1, a.n2, a.n3);
-  if (fi == 8) sm_fused_kernel<TX, TY, bf16, C, 8, EPI><<<nb, kThreads, 0, st>>>(a);
-  else if (fi == 4) sm_fused_kernel<TX, TY, bf16, C, 4, EPI><<<nb, kThreads, 0, st>>>(a);
+  const unsigned nb = fused_blocks(C, a.n1, a.n2, a.n3);
+  const int fi = fused_fi(C, a.n1, a.n2, a.n3);
+  if constexpr ((C + 1) * tile_cells(8) * 2 <= kMaxFusedLds) {
+    if (fi == 8) return sm_fused_kernel<TX, TY, bf16, C, 8, EPI><<<nb, kThreads, 0, st>>>(a);
+  }
+  if (fi == 4) sm_fused_kernel<TX, TY, bf16, C, 4, EPI><<<nb, kThreads, 0, st>>>(a);
   else sm_fused_kernel<TX, TY, bf16, C, 2, EPI><<<nb, kThreads, 0, st>>>(a);
 }
 
@@ -580,7 +592,8 @@ int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& args, hipStr
 #define MLPCG_FU(TXX, TYY)                                 \
   do {                                                     \
     if (C == 1) launch_fused_epi<TXX, TYY, 1>(a, st);      \
-    else launch_fused_epi<TXX, TYY, 2>(a, st);             \
+    else if (C == 2) launch_fused_epi<TXX, TYY, 2>(a, st); \
+    else launch_fused_epi<TXX, TYY, 3>(a, st);             \
   } while (0)
   if (tx == kF32 && ty == kBF16) MLPCG_FU(float, bf16);
   else if (tx == kBF16 && ty == kF32) MLPCG_FU(bf16, float);

@@ -39,7 +39,8 @@ struct UpArgs {
 
 // Fused apply for BF16 plans (fused.hip): K recomputed on the matrix cores from the FP16
 // image img [C][V] and the packed weights (launch_fused_pack), applied to x with the same
-// epilogue as ApplyArgs.  Grids with n1, n2 % 8 == 0 and n3 % 64 == 0, B = 1, C <= 2.
+// epilogue as ApplyArgs.  Grids with n1, n2 % 8 == 0 and n3 % 64 == 0, B = 1, C <= 3
+// (fused_eligible); fused_blocks(C, ...) workgroups per launch.
 struct FusedArgs {
   const void* img;
   const void* apack;  // packed weights and bias (launch_fused_pack)
@@ -61,7 +62,7 @@ struct FusedArgs {
 };
 
 size_t fused_weights_bytes(int C);
-unsigned fused_blocks(int n1, int n2, int n3);
+unsigned fused_blocks(int C, int n1, int n2, int n3);
 bool fused_eligible(int C, int B, int n1, int n2, int n3);
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st);
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream_t st);

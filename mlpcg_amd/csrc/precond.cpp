@@ -332,11 +332,11 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
       if (!swap) {
         if (!(tail && i == n - 1)) TRY(up(cur, tmp[lvl], lvl, 1.0, nullptr, nullptr, nullptr));
         if (last && dot0 && partials && dot_blocks && blocks[n].fused &&
-            2L * fused_blocks(dims[0][0], dims[0][1], dims[0][2]) <= reduction_capacity()) {
+            2L * fused_blocks(C, dims[0][0], dims[0][1], dims[0][2]) <= reduction_capacity()) {
           // the solver's dot products on z ride in the epilogue of the last block
           TRY(run_block(n, 0, tmp[0], dtype, y, ty, xd[0], c0p(0, 0), c1p(0, 0), ma, mb, nullptr, done, st,
                         dot0, dot1, partials));
-          *dot_blocks = fused_blocks(dims[0][0], dims[0][1], dims[0][2]);
+          *dot_blocks = fused_blocks(C, dims[0][0], dims[0][1], dims[0][2]);
           cur = xo[lvl];
           continue;
         }

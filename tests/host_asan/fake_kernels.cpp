@@ -432,7 +432,7 @@ int launch_sm_apply_transposed(int tk, const ApplyArgs& a, hipStream_t st) {
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream_t st) {
   MLPCG_REQUIRE(tw == kBF16 && fused_eligible(C, 1, a.n1, a.n2, a.n3), "sm_fused: not eligible");
   const size_t V = (size_t)a.n1 * a.n2 * a.n3;
-  const unsigned nb = fused_blocks(a.n1, a.n2, a.n3);
+  const unsigned nb = fused_blocks(C, a.n1, a.n2, a.n3);
   fakehip::enqueue(st, [=] {
     const char* who = "sm_fused";
     if (done_now(a.done)) return;

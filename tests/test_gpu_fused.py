@@ -22,9 +22,10 @@ def _plan(m, img, dtype, fused):
 
 
 def _exact_model(n, C, mask, seed):
-    """Block weights in {-2..2}/4 (C=1) or {-1..1}/4 (C=2) and biases in {-2..2}/4, so every
-    stencil value is a multiple of 1/16 below 16 in magnitude: exact in BF16/FP16 and in
-    the split BF16 weights; c0 = 0.5, c1 = 1 exactly."""
+    """Block weights in {-2..2}/4 (C=1) or {-1..1}/4 (C=2, 3) and biases in {-2..2}/4, so
+    every stencil value is a multiple of 1/16 below 16 in magnitude (C=3: at most
+    81 * 3/16 + 1/2): exact in BF16/FP16 and in the split BF16 weights; c0 = 0.5, c1 = 1
+    exactly."""
     from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
     g = torch.Generator().manual_seed(seed)
     m = SmallSMModelDn3D(n, C, "trilinear", mask, False)
@@ -67,6 +68,9 @@ def _nrel(a, b):
     (2, True, 1, (16, 8, 64)),      # two channels, model mask
     (2, False, 2, (32, 16, 128)),
     (1, False, 1, (256, 128, 256)),  # 2048 tiles: 8-plane tiles on level 0, 4-plane on level 1
+    (3, False, 2, (16, 16, 128)),   # three channels (the reference's image): 2-plane tiles
+    (3, True, 1, (16, 8, 64)),      # three channels, model mask (fluid channel)
+    (3, False, 1, (256, 128, 256)),  # three channels: tiles capped at 4 planes (LDS)
 ])
 def test_fused_matches_stored_stencils(dev, C, mask, n, shape):
     """With exactly representable weights, images and input, both BF16 plans compute the

@@ -621,7 +621,7 @@ int launch_cast(int tin, int tout, const void* in, void* out, long n, hipStream_
 }
 
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
-  MLPCG_REQUIRE(tw == kBF16 && (C == 1 || C == 2), "fused_pack: not eligible");
+  MLPCG_REQUIRE(tw == kBF16 && C >= 1 && C <= 3, "fused_pack: not eligible");
   fakehip::enqueue(st, [=] {
     const char* who = "fused_pack";
     RD(W, 27 * 27 * C * 4);

@@ -293,7 +293,7 @@ void plan_cases() {
                     make_plan(kind, g.n, C, g.g1, g.g2, g.g3, dtype, swap, mask, swap, W, &img, fused_mode, tail);
                 if (!P) continue;
                 const int nfused = mlpcg_precond_fused_blocks(P);
-                EXPECT(nfused >= 0 && (nfused == 0 || (dtype == MLPCG_BF16 && fused_mode > 0 && C <= 2)),
+                EXPECT(nfused >= 0 && (nfused == 0 || (dtype == MLPCG_BF16 && fused_mode > 0)),
                        "fused blocks %d in a plan of dtype %d, mode %d, C %d", nfused, dtype, fused_mode, C);
                 EXPECT(!mlpcg_precond_coarse_tail(P) || (tail && kind == 0 && !swap),
                        "coarse tail on a plan that did not ask for it");

@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 METRIC = "PCG time-to-1e-6 rel. residual at 256³, BF16 vs FP32 precond; train steps/s"
 HBM_PEAK_GBS = 8000.0
 DEFAULT_WEIGHTS = os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c1.pt")
+C3_WEIGHTS = os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c3.pt")
 
 
 def log(*a):
@@ -72,6 +73,7 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-train", action="store_true")
     ap.add_argument("--skip-fp16", action="store_true")
+    ap.add_argument("--skip-c3", action="store_true", help="skip the three-channel model solves")
     ap.add_argument("--roofline-iters", type=int, default=20)
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the dominant kernel (for rocprofv3 counter passes)")
@@ -98,18 +100,49 @@ def timed(fn):
     return time.perf_counter() - t0, out
 
 
-def load_model(path, levels, dev):
+def load_model(path, levels, dev, num_imgs=1):
     import torch
     from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
-    model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
+    model = SmallSMModelDn3D(levels, num_imgs, "trilinear").to(dev)
     source = "random-init"
     if path and os.path.exists(path):
         model.load_state_dict(torch.load(path, map_location=dev, weights_only=True))
         source = os.path.relpath(path, ROOT)
     else:
         torch.manual_seed(0)
-        model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
+        model = SmallSMModelDn3D(levels, num_imgs, "trilinear").to(dev)
     return model.eval(), source
+
+
+def three_channel_solves(args, sc, solver, b, dev):
+    """The same scene and RHS with a three-channel model ([air, fluid, non-solid] images,
+    the reference's own models): BF16 with the fused matrix-core apply, BF16 with stored
+    stencils, FP32; npsd to the same tolerance."""
+    import torch
+    from mlpcg_amd import scenes as S
+    model, wsrc = load_model(C3_WEIGHTS, args.levels, dev, num_imgs=3)
+    image = torch.as_tensor(S.image_channels(sc.flags, 3), device=dev)
+    out = {"weights": wsrc}
+    for name, dtype, fused in (("bf16_fused", torch.bfloat16, 1), ("bf16_stored", torch.bfloat16, 0),
+                               ("fp32", torch.float32, 0)):
+        plan = model.make_preconditioner(image, dtype, fused_mode=fused)
+
+        def step():
+            plan.update(image)
+            return solver.solve(b, plan, method=args.method, max_it=args.max_it, tol=args.tol)
+        for _ in range(args.warmup):
+            step()
+        times, info = [], None
+        for _ in range(args.steps):
+            t, (_, info) = timed(step)
+            times.append(t)
+        out[name] = {"s_per_solve": round(float(np.mean(times)), 6), "iters": info["iters"],
+                     "status": info["status"], "rel_res_device_fp64": info["rel_res"],
+                     "plan_mib": round(plan.nbytes / 2 ** 20, 1), "fused_blocks": plan.fused_blocks}
+        log(f"[bench] three channels {name}: {out[name]['s_per_solve'] * 1e3:.2f} ms/solve, "
+            f"{info['iters']} iterations, status {info['status']}, plan {out[name]['plan_mib']} MiB")
+        del plan
+    return out
 
 
 def roofline_apply(model, image, r, dtype, iters):
@@ -444,6 +477,9 @@ def main():
         return out
 
     results = run_method(args.method, list(plans))
+    c3 = None
+    if not args.skip_c3 and os.path.exists(C3_WEIGHTS):
+        c3 = three_channel_solves(args, sc, solver, b, dev)
     others = {}
     for m in [t for t in args.also.split(",") if t and t != args.method]:
         o = run_method(m, [n for n in ("bf16", "fp32") if n in plans])
@@ -537,6 +573,7 @@ def main():
         "iter_ratio_bf16_vs_fp32": round(bf["iters"] / max(f32["iters"], 1), 3),
         "solves": results,
         "other_methods": others,
+        "three_channel": c3,
         "train": train,
     }
     print(json.dumps(out), flush=True)

@@ -120,30 +120,36 @@ def test_fused_plan_vs_fp64_trained(dev):
     assert ef < 2.5 * es + 1e-4, (ef, es)
 
 
-def test_fused_bf16_solve_iterations(dev):
-    """npsd on the held-out 128^3 sphere scene of the shipped weights: the fused BF16 plan
-    converges (FP64-checked residual) within 1.1x the iterations of the FP32 plan."""
+@pytest.mark.parametrize("C", [1, 3])
+def test_fused_bf16_solve_iterations(dev, C):
+    """npsd on the held-out 128^3 sphere scene of the shipped weights (diag(A)/6 image, or
+    the three binary channels [air, fluid, non-solid]): the fused BF16 plan converges
+    (FP64-checked residual) within 1.1x the iterations of the FP32 plan, and the stored-K
+    BF16 plan within 1.1x too."""
     from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
     from mlpcg_amd.solver import FusedPCG
-    sd = torch.load(os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c1.pt"), weights_only=True)
+    sd = torch.load(os.path.join(ROOT, "mlpcg_amd", "weights", f"sm_l5_c{C}.pt"), weights_only=True)
     N = 128
-    m = SmallSMModelDn3D(5, 1, "trilinear").cuda()
+    m = SmallSMModelDn3D(5, C, "trilinear").cuda()
     m.load_state_dict(sd)
     sc = S.scene_spheres(N, 7)
     A = S.build_matrix(sc.flags)
     b = S.make_rhs(sc.flags, 0)[0]
-    img = torch.as_tensor(S.image_channels(sc.flags, 1)).float().cuda()
+    img = torch.as_tensor(S.image_channels(sc.flags, C)).float().cuda()
     s = FusedPCG(sc.flags, "flags", device=dev)
     bv = s.vector(b)
     its = {}
-    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
-        plan = _plan(m, img, dt, True)
+    for name, dt, fused in (("fp32", torch.float32, True), ("bf16", torch.bfloat16, True),
+                            ("bf16_stored", torch.bfloat16, False)):
+        plan = _plan(m, img, dt, fused)
+        assert (plan.fused_blocks > 0) == (name == "bf16")
         x, info = s.solve(bv, plan, method="npsd", max_it=300, tol=1e-6)
         assert info["status"] == 0, (name, info)
         xs = x.cpu().numpy()[sc.fluid_cells].astype(np.float64)
         assert np.linalg.norm(b - A @ xs) / np.linalg.norm(b) < 1.05e-6
         its[name] = info["iters"]
     assert its["bf16"] <= 1.1 * its["fp32"] + 1, its
+    assert its["bf16_stored"] <= 1.1 * its["fp32"] + 1, its
 
 
 @pytest.mark.parametrize("method", ["npcg", "npcg_flex", "npsd"])

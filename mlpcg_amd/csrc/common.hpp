@@ -88,6 +88,26 @@ __device__ __forceinline__ void load_vec_nt(const T* __restrict__ p, A* out) {
   }
 }
 
+// Non-temporal stores for outputs no kernel re-reads before they have left L2 (the PCG
+// vectors written once per iteration): they do not displace the reused rows from L2.
+template <int N, typename A, typename T>
+__device__ __forceinline__ void store_vec_nt(T* __restrict__ p, const A* in) {
+  constexpr int E = Piece<T, N>::kElems;
+  if constexpr (E >= 2) {
+    typedef T vt __attribute__((ext_vector_type(E)));
+#pragma unroll
+    for (int c = 0; c < N / E; ++c) {
+      vt v;
+#pragma unroll
+      for (int t = 0; t < E; ++t) v[t] = (T)in[c * E + t];
+      __builtin_nontemporal_store(v, reinterpret_cast<vt*>(p + c * E));
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < N; ++t) __builtin_nontemporal_store((T)in[t], p + t);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // XCD-aware block order.  Consecutive workgroup ids are dealt round-robin over the
 // 8 XCDs; remapping gives every XCD a contiguous range of tiles so the halo rows that

@@ -133,8 +133,9 @@ __global__ __launch_bounds__(kThreads) void resid_flat_kernel(FlatArgs a, int in
     double c[VZ + 2], s[4][VZ];
     load_star<VZ, double>(a.x_old, pp, alpha, r, a.n1, a.n2, a.n3, c, s);
     float bv[VZ], code[VZ];
-    load_vec<VZ>(a.b + r.v0, bv);
-    load_vec<VZ>(a.code + r.v0, code);
+    // single-use streams and outputs bypass L2 (the x and p rows of the neighbourhood stay)
+    load_vec_nt<VZ>(a.b + r.v0, bv);
+    load_vec_nt<VZ>(a.code + r.v0, code);
     float rv[VZ], xn[VZ];
 #pragma unroll
     for (int t = 0; t < VZ; ++t) {
@@ -145,8 +146,8 @@ __global__ __launch_bounds__(kThreads) void resid_flat_kernel(FlatArgs a, int in
       red[0] += res * res;
       red[1] += (double)bv[t] * (double)bv[t];
     }
-    store_vec<VZ>(a.r + r.v0, rv);
-    if (!init) store_vec<VZ>(a.x_new + r.v0, xn);
+    store_vec_nt<VZ>(a.r + r.v0, rv);
+    if (!init) store_vec_nt<VZ>(a.x_new + r.v0, xn);
   }
   block_partials<2>(red, a.partials);
 }
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(kThreads) void p_spmv_flat_kernel(FlatArgs a) {
     // p_new = z + beta * p_old, evaluated on the neighbourhood
     load_star<VZ, float>(a.z, a.p_old, beta, r, a.n1, a.n2, a.n3, c, s);
     float code[VZ];
-    load_vec<VZ>(a.code + r.v0, code);
+    load_vec_nt<VZ>(a.code + r.v0, code);
     float wv[VZ], pn[VZ];
 #pragma unroll
     for (int t = 0; t < VZ; ++t) {
@@ -177,8 +178,8 @@ __global__ __launch_bounds__(kThreads) void p_spmv_flat_kernel(FlatArgs a) {
       wv[t] = w;
       red[0] += (double)pn[t] * (double)w;
     }
-    store_vec<VZ>(a.p_new + r.v0, pn);
-    store_vec<VZ>(a.w + r.v0, wv);
+    store_vec_nt<VZ>(a.p_new + r.v0, pn);
+    store_vec_nt<VZ>(a.w + r.v0, wv);
   }
   block_partials<1>(red, a.partials);
 }
@@ -471,8 +472,9 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_flat_kernel(FlatArgs a, con
     float c[VZ + 2], s[4][VZ], nv[VZ];
     load_dir_star<VZ>(d, r, a.n1, a.n2, a.n3, c, s, nv);
     float code[VZ], rv[VZ], wv[VZ], qn[VZ];
-    load_vec<VZ>(a.code + r.v0, code);
-    load_vec<VZ>(a.r + r.v0, rv);
+    // single-use streams and outputs bypass L2 (the Q rows of the neighbourhood stay)
+    load_vec_nt<VZ>(a.code + r.v0, code);
+    load_vec_nt<VZ>(a.r + r.v0, rv);
 #pragma unroll
     for (int t = 0; t < VZ; ++t) {
       const int cd = (int)code[t];
@@ -482,8 +484,8 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_flat_kernel(FlatArgs a, con
       red[1] += (double)qn[t] * (double)rv[t];
       if (pN) red[2] += (double)nv[t] * (double)wv[t];
     }
-    store_vec<VZ>(a.p_new + r.v0, qn);
-    store_vec<VZ>(a.w + r.v0, wv);
+    store_vec_nt<VZ>(a.p_new + r.v0, qn);
+    store_vec_nt<VZ>(a.w + r.v0, wv);
   }
   block_partials<3>(red, a.partials);
 }

@@ -17,6 +17,10 @@ profiles/sm_fused_bf16_level0_pillars256_hbm_traffic.txt and
 profiles/sm_apply_bf16_level0_pillars256_hbm_traffic.txt for the 256^3 workload only
 (null otherwise).
 
+three_channel: the same scene solved with the three-channel model (image channels
+[air, fluid, non-solid], mlpcg_amd/weights/sm_l5_c3.pt) by the BF16 fused plan, the BF16
+stored-stencil plan and the FP32 plan (--skip-c3 leaves it out).
+
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 from __future__ import annotations

@@ -30,6 +30,10 @@ static size_t elem_size(int dt) { return dt == kF64 ? 8 : (dt == kF32 ? 4 : 2); 
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+// BF16 block forwards on grids of at least this many voxels use the fused matrix-core
+// apply (as the BF16 plans do); smaller grids stream stored stencils.
+constexpr size_t kFusedForwardMinVoxels = (size_t)1 << 18;
+
 }  // namespace mlpcg
 
 using namespace mlpcg;
@@ -72,6 +76,28 @@ static int block_fwd(bool trans, const void* image, const void* x, const void* w
   MLPCG_REQUIRE(image && x && weight && bias && y && workspace, "%s: null pointer", who);
   MLPCG_REQUIRE(aligned16(x) && aligned16(y) && aligned16(workspace) && aligned16(image),
                 "%s: pointers must be 16-byte aligned", who);
+  const size_t V = (size_t)n1 * n2 * n3;
+  if (!trans && dtype == kBF16 && V >= kFusedForwardMinVoxels && fused_eligible(C, B, n1, n2, n3)) {
+    // BF16 block: K recomputed on the matrix cores (the fused apply of the BF16 plans)
+    // from an FP16 copy of the image (exact for BF16 image values in the FP16 normal
+    // range) and the FP32 weights packed for it; no stencils are stored.  K, x and the
+    // sums round as in the stored-stencil apply (K to BF16, FP32 sums).
+    char* ws = (char*)workspace;
+    void* img16 = ws;
+    void* wpack = ws + ((C * V * 2 + 255) & ~(size_t)255);
+    if (launch_cast(kBF16, kF16, image, img16, (long)(C * V), STREAM(stream))) return -1;
+    if (launch_fused_pack(kBF16, weight, bias, C, wpack, STREAM(stream))) return -1;
+    FusedArgs f;
+    memset(&f, 0, sizeof(f));
+    f.img = img16;
+    f.apack = wpack;
+    f.x = x;
+    f.y = y;
+    f.n1 = n1;
+    f.n2 = n2;
+    f.n3 = n3;
+    return launch_sm_fused(kBF16, kBF16, kBF16, C, f, STREAM(stream));
+  }
   if (launch_stencil_weights(dtype, image, weight, bias, workspace, C, n1, n2, n3, trans, STREAM(stream)))
     return -1;
   ApplyArgs a;

@@ -180,7 +180,9 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
                                                               f16* __restrict__ apack) {
   const int NK = nkc(C);
   const int total = 2 * 2 * NK * 64 * 8;
-  for (int e = threadIdx.x; e < 2 * total; e += blockDim.x) {
+  // one element per thread over the grid: the two packings, then the bias accumulators
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < 2 * total) {
     const int j = e % 8, lane = (e / 8) % 64, kc = (e / 512) % NK, h = (e / (512 * NK)) % 2;
     const int par = (e / (1024 * NK)) % 2, form = e / total;
     const int rho = 16 * h + lane % 16, gk = lane / 16;
@@ -192,13 +194,13 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
       v = j == 6 ? hi : b[p] - hi;
     }
     apack[e] = (f16)v;
-  }
-  float* bpack = (float*)((char*)apack + 2 * fused_apack_bytes(C));
-  for (int e = threadIdx.x; e < 2 * 2 * 64 * 4; e += blockDim.x) {
-    const int i = e % 4, lane = (e / 4) % 64, h = (e / 256) % 2, par = e / 512;
+  } else if (e < 2 * total + 2 * 2 * 64 * 4) {
+    float* bpack = (float*)((char*)apack + 2 * fused_apack_bytes(C));
+    const int eb = e - 2 * total;
+    const int i = eb % 4, lane = (eb / 4) % 64, h = (eb / 256) % 2, par = eb / 512;
     const int rho = 16 * h + 4 * (lane / 16) + i;
     const bool rok = fused_slot_valid(par, row_g(rho), row_j(rho));
-    bpack[e] = rok ? b[fused_tap(par, row_g(rho), row_j(rho))] : 0.f;
+    bpack[eb] = rok ? b[fused_tap(par, row_g(rho), row_j(rho))] : 0.f;
   }
 }
 
@@ -543,7 +545,9 @@ bool fused_eligible(int C, int B, int n1, int n2, int n3) {
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
   MLPCG_REQUIRE(C >= 1 && C <= 3, "fused_pack: C must be 1, 2 or 3");
   MLPCG_REQUIRE(tw == kBF16, "fused_pack: BF16 plans only (got %d)", tw);
-  fused_pack_kernel<<<1, kThreads, 0, st>>>((const float*)W, (const float*)b, C, (f16*)wbuf);
+  const int n = 2 * (2 * 2 * nkc(C) * 64 * 8) + 2 * 2 * 64 * 4;
+  fused_pack_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>((const float*)W, (const float*)b, C,
+                                                                       (f16*)wbuf);
   return check_launch("fused_pack");
 }
 

@@ -24,6 +24,21 @@ __global__ __launch_bounds__(kThreads) void cast_kernel(const TI* __restrict__ i
   if (i < n) out[i] = (TO)(typename AccOf<TO>::type)in[i];
 }
 
+// 16-bit to 16-bit, 8 elements (16 bytes) per thread: n % 8 == 0, 16-byte aligned buffers
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kThreads) void cast8_kernel(const TI* __restrict__ in, TO* __restrict__ out,
+                                                         long n8) {
+  typedef TI vi __attribute__((ext_vector_type(8)));
+  typedef TO vo __attribute__((ext_vector_type(8)));
+  const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n8) return;
+  const vi a = ((const vi*)in)[i];
+  vo b;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) b[k] = (TO)(float)a[k];
+  ((vo*)out)[i] = b;
+}
+
 int launch_nonzero_mask(int acc, const void* ch, uint8_t* m, long V, hipStream_t st) {
   const unsigned nb = (unsigned)((V + kThreads - 1) / kThreads);
   if (acc == kF64) nonzero_mask_kernel<double><<<nb, kThreads, 0, st>>>((const double*)ch, m, V);
@@ -38,6 +53,12 @@ int launch_square(int acc, const void* in, void* out, int n, hipStream_t st) {
 }
 
 int launch_cast(int tin, int tout, const void* in, void* out, long n, hipStream_t st) {
+  if (tin == kBF16 && tout == kF16 && n % 8 == 0 && ((uintptr_t)in & 15u) == 0 && ((uintptr_t)out & 15u) == 0) {
+    const long n8 = n / 8;
+    cast8_kernel<bf16, f16><<<(unsigned)((n8 + kThreads - 1) / kThreads), kThreads, 0, st>>>((const bf16*)in,
+                                                                                          (f16*)out, n8);
+    return check_launch("cast");
+  }
   const unsigned nb = (unsigned)((n + kThreads - 1) / kThreads);
 #define MLPCG_CAST(TI_, TIT, TO_, TOT)                                                         \
   if (tin == TI_ && tout == TO_) {                                                             \
@@ -49,6 +70,7 @@ int launch_cast(int tin, int tout, const void* in, void* out, long n, hipStream_
   MLPCG_CAST(kF32, float, kF32, float)
   MLPCG_CAST(kF32, float, kF64, double)
   MLPCG_CAST(kBF16, bf16, kF32, float)
+  MLPCG_CAST(kBF16, bf16, kF16, f16)
   MLPCG_CAST(kF16, f16, kF32, float)
   MLPCG_CAST(kF64, double, kF32, float)
   MLPCG_CAST(kF64, double, kF64, double)

@@ -531,6 +531,274 @@ __global__ __launch_bounds__(kThreads) void weight_grad_mfma_lds_kernel(
   }
 }
 
+// BF16 weight gradient on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16), B == 1,
+// n3 % 64 == 0, any C <= 3.  G_p(v) = a(v) c(v + d_p) of two BF16 values has at most 16
+// significant bits, so it is exact in FP32 and splits exactly into hi + lo BF16 parts
+// (hi: the top 8 bits by truncation, lo = G - hi: the remaining <= 8 bits).  With
+// D[p][q] = sum_v hi_p(v) P_q(v) + lo_p(v) P_q(v) every product is exact and the sums are
+// FP32, as in the FP32-operand kernels above; the matrix cores do 32 voxels per product
+// instead of 4.
+//
+// A wave walks a column of z-row segments (plane i, rows j0 .. j0 + J - 1, 64 z cells,
+// two K chunks of 32 voxels) and keeps the three neighbour rows along j of each plane
+// i - 1, i, i + 1 in a ring of LDS slots, so each step stages one new row per plane
+// (loaded into registers during the previous step's products).  x rows are stored as FP32
+// (one copy; a lane's 8 taps along z are read at 4-byte alignment), image rows as three
+// BF16 copies shifted by one cell so that every B operand is one aligned 16-byte read.
+// Operand layout: A[p][k] lane 16 kb + m holds row m, voxels 8 kb .. 8 kb + 7; B[k][q]
+// lane 16 kb + m holds column m of the same voxels; D[4 kb + i][m].  Rows p = 27..31 and
+// columns q > 27C read constant zero rows, column 27C (bias) a row of ones.
+namespace wgb {
+constexpr int kSeg = 64;                       // z cells per row segment
+constexpr int kXS = 72;                        // FP32 x row: cell z0 - 1 + e at element 3 + e
+constexpr int kIS = 72;                        // BF16 image row copy (64 cells used)
+constexpr int kXBytes = 4 * 9 * kXS * 4;       // x ring slots of the 4 waves
+constexpr int kGBytes = 4 * kSeg * 4;          // a rows (FP32) of the 4 waves
+constexpr int kZBytes = kXS * 4;               // FP32 zero row
+constexpr int kCBytes = 2 * kIS * 2;           // BF16 zero row, ones row
+__host__ __device__ constexpr int img_bytes(int C) { return 4 * C * 27 * kIS * 2; }
+__host__ __device__ constexpr int lds_bytes(int C) { return kXBytes + kGBytes + kZBytes + kCBytes + img_bytes(C); }
+}  // namespace wgb
+
+typedef __bf16 wg_bf16x8 __attribute__((ext_vector_type(8), aligned(16)));
+typedef uint32_t wg_u32x4 __attribute__((ext_vector_type(4)));
+
+// hi / lo BF16 parts of an exact FP32 product, packed in pairs (element 0 in the low half)
+__device__ __forceinline__ void wg_split2(float p0, float p1, uint32_t& hi, uint32_t& lo) {
+  const uint32_t u0 = __float_as_uint(p0), u1 = __float_as_uint(p1);
+  const float l0 = p0 - __uint_as_float(u0 & 0xffff0000u);
+  const float l1 = p1 - __uint_as_float(u1 & 0xffff0000u);
+  hi = __builtin_amdgcn_perm(u1, u0, 0x07060302u);
+  lo = __builtin_amdgcn_perm(__float_as_uint(l1), __float_as_uint(l0), 0x07060302u);
+}
+
+template <int C>
+__global__ __launch_bounds__(kThreads) void weight_grad_bf16_kernel(
+    const bf16* __restrict__ a, const bf16* __restrict__ cv, const bf16* __restrict__ img, int n1,
+    int n2, int n3, int J, float* __restrict__ partials) {
+  using namespace wgb;
+  constexpr int NQ = 27 * C + 1, NT = (NQ + 15) / 16, NO = 27 * NQ;
+  constexpr int NTASK = 24 + 24 * C + 8, NR = (NTASK + 63) / 64;  // staging tasks per step
+  static_assert(4 * 27 * NQ * 4 <= img_bytes(C), "reduction buffer");
+  __shared__ __attribute__((aligned(16))) char lds[lds_bytes(C)];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, kb = lane >> 4;
+  // byte offsets in lds
+  const int oX = w * 9 * kXS * 4;
+  const int oG = kXBytes + w * kSeg * 4;
+  const int oZ = kXBytes + kGBytes;
+  const int oC = oZ + kZBytes;
+  const int oI = oC + kCBytes + w * C * 27 * kIS * 2;
+  for (int e = threadIdx.x; e < kXS; e += kThreads) *(float*)(lds + oZ + 4 * e) = 0.f;
+  for (int e = threadIdx.x; e < 2 * kIS; e += kThreads)
+    *(uint16_t*)(lds + oC + 2 * e) = e < kIS ? (uint16_t)0 : (uint16_t)0x3f80;  // BF16 0, 1
+  __syncthreads();
+
+  // this lane's operand rows: A taps p = 16 h + m, B columns q = 16 gt + m.  Offset at
+  // step j = base + slot * stride with slot = (j + 2 + e2) mod 3 the ring slot of row
+  // j + e2 - 1 (e2 = d2 + 1); constant rows have stride 0.
+  int abase[2], ae2[2], astr[2], bbase[NT], be2[NT], bstr[NT];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int p = 16 * h + m;
+    if (p < 27) {
+      abase[h] = oX + ((p / 9) * 3 * kXS + 3 + p % 3 + 8 * kb) * 4;
+      ae2[h] = (p / 3) % 3;
+      astr[h] = kXS * 4;
+    } else {
+      abase[h] = oZ + 8 * kb * 4;
+      ae2[h] = 0;
+      astr[h] = 0;
+    }
+  }
+#pragma unroll
+  for (int gt = 0; gt < NT; ++gt) {
+    const int q = 16 * gt + m;
+    if (q < 27 * C) {
+      const int c = q / 27, s = q % 27;
+      bbase[gt] = oI + (((c * 9 + (s / 9) * 3) * 3 + s % 3) * kIS + 8 * kb) * 2;
+      be2[gt] = (s / 3) % 3;
+      bstr[gt] = 3 * kIS * 2;
+    } else {
+      bbase[gt] = oC + ((q == 27 * C ? kIS : 0) + 8 * kb) * 2;
+      be2[gt] = 0;
+      bstr[gt] = 0;
+    }
+  }
+
+  const long V = (long)n1 * n2 * n3;
+  const int nseg3 = n3 / kSeg, njb = (n2 + J - 1) / J;
+  const long ncol = (long)n1 * nseg3 * njb;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  f4 acc[2][NT];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int gt = 0; gt < NT; ++gt) acc[h][gt] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // staging task e of a step: e < 24: x row (plane i + e / 8 - 1, row jr), chunk e % 8;
+  // e < 24 + 24C: image channel, same rows; else the a row (i, ja), chunk e % 8.  A task
+  // loads the chunk's 8 cells plus the cell pairs before and after it (zero outside the grid).
+  struct Regs {
+    uint32_t d[NR][6];
+  };
+  auto task_src = [&](int e, int i, int jr, int ja, int z0, const bf16*& src, bool& ok, int& t) {
+    int ii, jj;
+    if (e < 24) {
+      src = cv;
+      ii = i + e / 8 - 1;
+      jj = jr;
+    } else if (e < 24 + 24 * C) {
+      const int f = e - 24;
+      src = img + (long)(f / 24) * V;
+      ii = i + (f % 24) / 8 - 1;
+      jj = jr;
+    } else {
+      src = a;
+      ii = i;
+      jj = ja;
+    }
+    t = e & 7;
+    ok = e < NTASK && (unsigned)ii < (unsigned)n1 && (unsigned)jj < (unsigned)n2;
+    src += ((long)(ok ? ii : 0) * n2 + (ok ? jj : 0)) * n3 + z0 + 8 * t;
+  };
+  auto load_step = [&](Regs& r, int i, int jr, int ja, int z0) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int e = lane + 64 * k;
+      const bf16* src;
+      bool ok;
+      int t;
+      task_src(e, i, jr, ja, z0, src, ok, t);
+      const uint32_t* s32 = (const uint32_t*)src;
+      const wg_u32x4 v = ok ? *(const wg_u32x4*)s32 : wg_u32x4{0u, 0u, 0u, 0u};
+      const bool okp = ok && z0 + 8 * t > 0, okn = ok && z0 + 8 * t + 8 < n3;
+      r.d[k][0] = okp ? s32[-1] : 0u;
+      r.d[k][1] = v[0];
+      r.d[k][2] = v[1];
+      r.d[k][3] = v[2];
+      r.d[k][4] = v[3];
+      r.d[k][5] = okn ? s32[4] : 0u;
+    }
+  };
+  // slot of row jr in the ring: (jr + 3) mod 3 (jr >= -1)
+  auto write_step = [&](const Regs& r, int jr) {
+    const int sl = (jr + 3) % 3;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int e = lane + 64 * k;
+      const int t = e & 7;
+      const uint32_t* d = r.d[k];
+      if (e < 24) {
+        char* dst = lds + oX + (((e / 8) * 3 + sl) * kXS + 4 + 8 * t) * 4;
+        *(f4*)dst = f4{__uint_as_float(d[1] << 16), __uint_as_float(d[1] & 0xffff0000u),
+                       __uint_as_float(d[2] << 16), __uint_as_float(d[2] & 0xffff0000u)};
+        *(f4*)(dst + 16) = f4{__uint_as_float(d[3] << 16), __uint_as_float(d[3] & 0xffff0000u),
+                              __uint_as_float(d[4] << 16), __uint_as_float(d[4] & 0xffff0000u)};
+        if (t == 0) *(float*)(dst - 4) = __uint_as_float(d[0] & 0xffff0000u);
+        if (t == 7) *(float*)(dst + 32) = __uint_as_float(d[5] << 16);
+      } else if (e < 24 + 24 * C) {
+        const int f = e - 24;
+        char* dst = lds + oI + ((((f / 24) * 9 + ((f % 24) / 8) * 3 + sl) * 3) * kIS + 8 * t) * 2;
+        *(wg_u32x4*)dst = wg_u32x4{__builtin_amdgcn_alignbyte(d[1], d[0], 2), __builtin_amdgcn_alignbyte(d[2], d[1], 2),
+                                   __builtin_amdgcn_alignbyte(d[3], d[2], 2), __builtin_amdgcn_alignbyte(d[4], d[3], 2)};
+        *(wg_u32x4*)(dst + kIS * 2) = wg_u32x4{d[1], d[2], d[3], d[4]};
+        *(wg_u32x4*)(dst + 2 * kIS * 2) =
+            wg_u32x4{__builtin_amdgcn_alignbyte(d[2], d[1], 2), __builtin_amdgcn_alignbyte(d[3], d[2], 2),
+                     __builtin_amdgcn_alignbyte(d[4], d[3], 2), __builtin_amdgcn_alignbyte(d[5], d[4], 2)};
+      } else if (e < NTASK) {
+        char* dst = lds + oG + 8 * t * 4;
+        *(f4*)dst = f4{__uint_as_float(d[1] << 16), __uint_as_float(d[1] & 0xffff0000u),
+                       __uint_as_float(d[2] << 16), __uint_as_float(d[2] & 0xffff0000u)};
+        *(f4*)(dst + 16) = f4{__uint_as_float(d[3] << 16), __uint_as_float(d[3] & 0xffff0000u),
+                              __uint_as_float(d[4] << 16), __uint_as_float(d[4] & 0xffff0000u)};
+      }
+    }
+  };
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+
+  for (long col = (long)blockIdx.x * 4 + w; col < ncol; col += (long)gridDim.x * 4) {
+    // column order: plane fastest, so the waves of a block share neighbour planes
+    const int i = (int)(col % n1);
+    const long rest = col / n1;
+    const int jb = (int)(rest % njb);
+    const int z0 = (int)(rest / njb) * kSeg;
+    const int j0 = jb * J, j1 = min(j0 + J, n2);
+    Regs r;
+    // prologue: rows j0 - 1 and j0 (the a-row tasks of these two are not used)
+    load_step(r, i, j0 - 1, j0, z0);
+    write_step(r, j0 - 1);
+    load_step(r, i, j0, j0, z0);
+    write_step(r, j0);
+    load_step(r, i, j0 + 1, j0, z0);
+    for (int j = j0; j < j1; ++j) {
+      write_step(r, j + 1);  // row j + 1 and the a row of j
+      wave_sync();
+      if (j + 1 < j1) load_step(r, i, j + 2, j + 1, z0);
+      const int sj = (j + 2) % 3;
+      int ao[2], bo[NT];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        int s = sj + ae2[h];
+        s = s >= 3 ? s - 3 : s;
+        ao[h] = abase[h] + s * astr[h];
+      }
+#pragma unroll
+      for (int gt = 0; gt < NT; ++gt) {
+        int s = sj + be2[gt];
+        s = s >= 3 ? s - 3 : s;
+        bo[gt] = bbase[gt] + s * bstr[gt];
+      }
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc) {
+        const f4 g0 = *(const f4*)(lds + oG + (32 * cc + 8 * kb) * 4);
+        const f4 g1 = *(const f4*)(lds + oG + (32 * cc + 8 * kb + 4) * 4);
+        const float gv[8] = {g0[0], g0[1], g0[2], g0[3], g1[0], g1[1], g1[2], g1[3]};
+        wg_bf16x8 ahi[2], alo[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const float* xp = (const float*)(lds + ao[h] + 128 * cc);
+          uint32_t hi[4], lo[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) wg_split2(gv[2 * k] * xp[2 * k], gv[2 * k + 1] * xp[2 * k + 1], hi[k], lo[k]);
+          ahi[h] = __builtin_bit_cast(wg_bf16x8, wg_u32x4{hi[0], hi[1], hi[2], hi[3]});
+          alo[h] = __builtin_bit_cast(wg_bf16x8, wg_u32x4{lo[0], lo[1], lo[2], lo[3]});
+        }
+#pragma unroll
+        for (int gt = 0; gt < NT; ++gt) {
+          const wg_bf16x8 bv = *(const wg_bf16x8*)__builtin_assume_aligned(lds + bo[gt] + 64 * cc, 16);
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            acc[h][gt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[h], bv, acc[h][gt], 0, 0, 0);
+            acc[h][gt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[h], bv, acc[h][gt], 0, 0, 0);
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+  // D[16 h + 4 kb + t][16 gt + m] of this wave -> LDS (over the image slots), then a
+  // fixed-order sum over the four waves
+  __syncthreads();
+  float* red = (float*)(lds + kXBytes + kGBytes + kZBytes + kCBytes);
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int gt = 0; gt < NT; ++gt)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int p = 16 * h + 4 * kb + t, q = 16 * gt + m;
+        if (p < 27 && q < NQ) red[(w * 27 + p) * NQ + q] = acc[h][gt][t];
+      }
+  __syncthreads();
+  for (int o = threadIdx.x; o < NO; o += kThreads)
+    partials[(size_t)blockIdx.x * NO + o] = (red[o] + red[NO + o]) + (red[2 * NO + o] + red[3 * NO + o]);
+}
+
 template <typename A>
 __global__ __launch_bounds__(kThreads) void weight_grad_final_kernel(const A* __restrict__ partials,
                                                                      int nblk, int C,
@@ -588,6 +856,20 @@ int launch_weight_grad(int dtype, const void* a, const void* c, const void* img,
                        void* workspace, int C, int B, int n1, int n2, int n3, hipStream_t st) {
   MLPCG_REQUIRE(C >= 1 && C <= 3, "weight_grad: C must be 1, 2 or 3 (got %d)", C);
   const int nblk = wg_blocks(dtype, C, n1, n2, n3);
+  if (dtype == kBF16 && B == 1 && n3 % wgb::kSeg == 0) {
+    // BF16 matrix-core kernel (same partials layout and block count): columns of J row
+    // segments, about one column per wave
+    const long segs = (long)n1 * n2 * (n3 / wgb::kSeg);
+    const long per_wave = (segs + 4L * nblk - 1) / (4L * nblk);
+    const int J = (int)(per_wave < 4 ? 4 : (per_wave > 64 ? 64 : per_wave));
+    const bf16 *pa = (const bf16*)a, *pc = (const bf16*)c, *pi = (const bf16*)img;
+    if (C == 1) weight_grad_bf16_kernel<1><<<nblk, kThreads, 0, st>>>(pa, pc, pi, n1, n2, n3, J, (float*)workspace);
+    else if (C == 2) weight_grad_bf16_kernel<2><<<nblk, kThreads, 0, st>>>(pa, pc, pi, n1, n2, n3, J, (float*)workspace);
+    else weight_grad_bf16_kernel<3><<<nblk, kThreads, 0, st>>>(pa, pc, pi, n1, n2, n3, J, (float*)workspace);
+    weight_grad_final_kernel<float><<<27 * (27 * C + 1), kThreads, 0, st>>>((const float*)workspace, nblk, C,
+                                                                           (float*)gW, (float*)gb);
+    return check_launch("weight_grad");
+  }
   if (C == 1 && dtype != kF64 && n3 % 4 == 0) {
     // matrix-core kernel (same partials layout and block count)
     switch (dtype) {

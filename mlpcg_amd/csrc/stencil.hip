@@ -257,6 +257,61 @@ __global__ __launch_bounds__(kThreads) void sm_apply_t_kernel(ApplyArgs a) {
   }
 }
 
+// The same gather (B = 1) without halo loads, where every z-row lies within one wave
+// (n3 / VZ runs, a power of two up to 64): per source row the thread forms the products
+// S_p(z') = K_p(z') x(z') on its own aligned run only (16-byte loads of K and x); tap
+// d3 moves them to z = z' + d3, so the run's edge products of taps d3 = -1 and +1 belong to
+// the neighbouring runs.  Those are summed over the nine source rows and handed over by
+// two DPP wave shifts at the end.  27 + 9 loads per run instead of 3 (27 + 9).
+template <typename TK, int VZ>
+__global__ __launch_bounds__(kThreads) void sm_apply_t_edge_kernel(ApplyArgs a) {
+  typedef typename AccOf<TK>::type A;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  const int L = a.n3 / VZ;  // runs per z-row
+  const long nch = (long)a.n1 * a.n2 * L;
+  if (cid >= nch) return;
+  const int kc = (int)(cid % L);
+  const long row = cid / L;
+  const int j = (int)(row % a.n2), i = (int)(row / a.n2);
+  const int k0 = kc * VZ;
+  const long V = (long)a.n1 * a.n2 * a.n3;
+  const TK* __restrict__ K = (const TK*)a.K;
+  const TK* __restrict__ x = (const TK*)a.x;
+  A acc[VZ];
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) acc[t] = A(0);
+  A to_prev = A(0), to_next = A(0);  // edge products for the runs before / after this one
+#pragma unroll
+  for (int a1 = 0; a1 < 3; ++a1) {
+#pragma unroll
+    for (int a2 = 0; a2 < 3; ++a2) {
+      const int ri = i + 1 - a1, rj = j + 1 - a2;  // source row (i - d1, j - d2)
+      if ((unsigned)ri >= (unsigned)a.n1 || (unsigned)rj >= (unsigned)a.n2) continue;
+      const long off = ((long)ri * a.n2 + rj) * a.n3 + k0;
+      A xv[VZ], k0v[VZ], k1v[VZ], k2v[VZ];
+      load_vec<VZ>(x + off, xv);
+      const long p0 = (long)(9 * a1 + 3 * a2) * V;
+      load_vec_nt<VZ>(K + p0 + off, k0v);
+      load_vec_nt<VZ>(K + p0 + V + off, k1v);
+      load_vec_nt<VZ>(K + p0 + 2 * V + off, k2v);
+      // tap d3 = 0 stays, d3 = -1 moves down one cell, d3 = +1 up one cell
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) acc[t] += k1v[t] * xv[t];
+#pragma unroll
+      for (int t = 1; t < VZ; ++t) acc[t - 1] += k0v[t] * xv[t];
+#pragma unroll
+      for (int t = 0; t + 1 < VZ; ++t) acc[t + 1] += k2v[t] * xv[t];
+      to_prev += k0v[0] * xv[0];
+      to_next += k2v[VZ - 1] * xv[VZ - 1];
+    }
+  }
+  // the row's runs are consecutive lanes of this wave: lane l + 1 holds the next run
+  const A from_next = wave_from_next(to_prev), from_prev = wave_from_prev(to_next);
+  if (kc + 1 < L) acc[VZ - 1] += from_next;
+  if (kc > 0) acc[0] += from_prev;
+  store_vec<VZ>((TK*)a.y + ((long)i * a.n2 + j) * a.n3 + k0, acc);
+}
+
 // ---------------------------------------------------------------------------
 // sm_apply
 // ---------------------------------------------------------------------------
@@ -658,6 +713,22 @@ int launch_sm_apply_transposed(int tk, const ApplyArgs& args, hipStream_t st) {
   // halos hold 256 VGPRs and the BF16 level-0 gather at 256^3 took 685 us against 534 us
   // (halo cells from neighbouring lanes by DPP shifts instead of 2-byte loads: 639 us with
   // 8 cells, 632 us with 4)
+  if (args.B == 1) {
+    // edge-exchange gather with 16-byte runs where each z-row fits one wave
+    const int ve = tk == kF64 ? 2 : (tk == kF32 ? 4 : 8);
+    const int L = args.n3 / ve;
+    if (args.n3 % ve == 0 && L <= 64 && 64 % L == 0) {
+      const dim3 g = grid1((long)args.n1 * args.n2 * L);
+      switch (tk) {
+        case kF64: sm_apply_t_edge_kernel<double, 2><<<g, kThreads, 0, st>>>(args); break;
+        case kF32: sm_apply_t_edge_kernel<float, 4><<<g, kThreads, 0, st>>>(args); break;
+        case kF16: sm_apply_t_edge_kernel<f16, 8><<<g, kThreads, 0, st>>>(args); break;
+        case kBF16: sm_apply_t_edge_kernel<bf16, 8><<<g, kThreads, 0, st>>>(args); break;
+        default: MLPCG_REQUIRE(false, "sm_apply_t: bad dtype %d", tk);
+      }
+      return check_launch("sm_apply_t");
+    }
+  }
   const int vz = pick_vz_small(args.n3, tk == kF64 ? 2 : 4, (long)args.n1 * args.n2 * args.n3 * args.B);
   switch (tk) {
     case kF64: launch_apply_tr_t<double>(args, vz, st); break;

@@ -96,6 +96,26 @@ def test_fused_matches_stored_stencils(dev, C, mask, n, shape):
     assert torch.all(zf2[om == 0] == 0)
 
 
+@pytest.mark.parametrize("C,shape", [(1, (64, 64, 64)), (2, (64, 32, 128)), (3, (64, 64, 64))])
+def test_block_forward_bf16_fused_exact(dev, C, shape):
+    """The BF16 block forward (training and inference) on grids of 2^18 voxels or more runs
+    the fused matrix-core apply.  With exactly representable weights, image and input every
+    stencil and every sum is exact on both paths, so y equals the FP64 forward rounded to
+    BF16 bit for bit: a wrong tap, halo, channel or weight packing shows at once."""
+    from mlpcg_amd import kernels as K
+    g = torch.Generator().manual_seed(11 + C)
+    span = 2 if C == 1 else 1
+    W = torch.randint(-span, span + 1, (27, C, 3, 3, 3), generator=g).double() / 4
+    b = torch.randint(-2, 3, (27,), generator=g).double() / 4
+    img = torch.randint(0, 4, (C, *shape), generator=g).double() / 4
+    x = torch.randint(-4, 5, (1, 1, *shape), generator=g).double()
+    y64 = K.sm_block_forward(img.to(dev), x.to(dev), W.to(dev), b.to(dev))
+    y16 = K.sm_block_forward(img.to(dev).bfloat16(), x.to(dev).bfloat16(), W.float().to(dev),
+                             b.float().to(dev))
+    assert y16.dtype == torch.bfloat16
+    assert torch.equal(y16, y64.bfloat16())
+
+
 def test_fused_plan_vs_fp64_trained(dev):
     """Trained weights on a 64^3 pillars scene: the fused BF16 plan is about as close to
     the FP64 oracle composition as the stored-K BF16 plan (both round the stencils to

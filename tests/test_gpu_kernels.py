@@ -96,7 +96,8 @@ def test_sm_block_forward_f32(dev, trans):
 
 
 @pytest.mark.parametrize("trans", [False, True])
-@pytest.mark.parametrize("C,B,shape", [(1, 2, (6, 5, 8)), (3, 1, (4, 6, 4)), (2, 3, (5, 4, 6))])
+@pytest.mark.parametrize("C,B,shape", [(1, 2, (6, 5, 8)), (3, 1, (4, 6, 4)), (2, 3, (5, 4, 6)),
+                                       (1, 1, (5, 4, 16)), (3, 1, (4, 3, 8))])
 def test_sm_block_backward_f64(dev, trans, C, B, shape):
     """Gradients vs torch autograd of the FP64 oracle."""
     from mlpcg_amd import kernels as K
@@ -139,7 +140,7 @@ def test_sm_block_backward_finite_differences_f64(dev, trans):
 
 @pytest.mark.parametrize("trans", [False, True])
 @pytest.mark.parametrize("B,shape", [(2, (6, 5, 8)), (1, (4, 6, 12)), (3, (3, 4, 20)), (1, (8, 8, 96)),
-                                     (2, (5, 4, 6))])
+                                     (2, (5, 4, 6)), (1, (6, 5, 32)), (1, (3, 4, 256))])
 def test_sm_block_backward_f32_vs_f64(dev, trans, B, shape):
     """One-channel FP32 backward (matrix-core weight gradient where n3 % 4 == 0, VALU
     otherwise) against the FP64 backward on the same inputs: dx, dW, db."""
@@ -153,6 +154,50 @@ def test_sm_block_backward_f32_vs_f64(dev, trans, B, shape):
     for o, r in zip(out, ref):
         assert o.dtype == torch.float32
         assert _rel(o, r) < 2e-6
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("C,shape", [(1, (8, 8, 64)), (1, (5, 7, 128)), (2, (4, 6, 64)), (3, (6, 5, 128)),
+                                     (3, (1, 3, 64)), (1, (64, 100, 256)), (3, (16, 16, 64))])
+def test_sm_block_backward_bf16_weight_grad(dev, trans, C, shape):
+    """BF16 weight gradient on the BF16 matrix cores (n3 % 64 == 0): with BF16 g, x and
+    image every product g x img is exact and the sums are FP32, so dW and db match the
+    FP64 backward of the same (BF16-valued) inputs to FP32 summation error.  The shapes
+    cover one-plane grids, ragged row columns (100 rows in columns of 7) and C = 1..3."""
+    from mlpcg_amd import kernels as K
+    img = _rand((C, *shape), 50).abs().to(torch.bfloat16)
+    x, g = _rand((1, 1, *shape), 51).to(torch.bfloat16), _rand((1, 1, *shape), 52).to(torch.bfloat16)
+    W, b = _rand((27, C, 3, 3, 3), 53) / 9, _rand((27,), 54) / 9
+    f = K.sm_block_trans_backward if trans else K.sm_block_backward
+    _, gW, gb = f(g.to(dev), img.to(dev), x.to(dev), W.float().to(dev), b.float().to(dev))
+    _, rW, rb = f(g.double().to(dev), img.double().to(dev), x.double().to(dev), W.to(dev), b.to(dev))
+    assert gW.dtype == torch.float32
+    assert _rel(gW, rW) < 1e-5
+    assert _rel(gb, rb) < 1e-5
+
+
+@pytest.mark.parametrize("trans", [False, True])
+@pytest.mark.parametrize("C,shape", [(1, (8, 8, 64)), (3, (5, 7, 128)), (1, (4, 6, 256)), (2, (2, 3, 512)),
+                                     (1, (3, 5, 8))])
+def test_sm_block_backward_bf16_dx_exact(dev, trans, C, shape):
+    """BF16 input gradient (the transposed gather, with z-rows of 1..64 runs per wave, and
+    the forward apply for the transpose block): with exactly representable weights,
+    image and output gradient every stencil and every sum is exact, so dx equals the FP64
+    backward rounded to BF16 bit for bit."""
+    from mlpcg_amd import kernels as K
+    g = torch.Generator().manual_seed(31 + C)
+    span = 2 if C == 1 else 1
+    W = torch.randint(-span, span + 1, (27, C, 3, 3, 3), generator=g).double() / 4
+    b = torch.randint(-2, 3, (27,), generator=g).double() / 4
+    img = torch.randint(0, 4, (C, *shape), generator=g).double() / 4
+    x = torch.randint(-4, 5, (1, 1, *shape), generator=g).double()
+    go = torch.randint(-4, 5, (1, 1, *shape), generator=g).double()
+    f = K.sm_block_trans_backward if trans else K.sm_block_backward
+    dx64, _, _ = f(go.to(dev), img.to(dev), x.to(dev), W.to(dev), b.to(dev))
+    dx16, _, _ = f(go.to(dev).bfloat16(), img.to(dev).bfloat16(), x.to(dev).bfloat16(), W.float().to(dev),
+                   b.float().to(dev))
+    assert dx16.dtype == torch.bfloat16
+    assert torch.equal(dx16, dx64.bfloat16())
 
 
 @pytest.mark.parametrize("C,shape", [(1, (8, 8, 8)), (3, (6, 10, 12)), (2, (1, 4, 5)), (1, (16, 16, 32))])

@@ -21,6 +21,10 @@ three_channel: the same scene solved with the three-channel model (image channel
 [air, fluid, non-solid], mlpcg_amd/weights/sm_l5_c3.pt) by the BF16 fused plan, the BF16
 stored-stencil plan and the FP32 plan (--skip-c3 leaves it out).
 
+train: Adam steps/s of the graph-captured training step on 8 x 128^3 scenes (FP32), and
+train.train256 that of the 4 x 256^3 training step (40-sphere scenes, shipped weights)
+with BF16 and with FP32 activations, and their ratio (--skip-train leaves both out).
+
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 from __future__ import annotations
@@ -332,6 +336,53 @@ def workload_spheres128(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def train256_scenes(dev):
+    from mlpcg_amd import train as T
+    return [T.SceneBatch(s, 1, dev) for s in T.make_scenes(256, range(300, 304), "spheres40")]
+
+
+def train256_steps(args, dev, scenes, act_name, steps, gen):
+    """`steps` Adam steps on the 4 training scenes at 256^3 from the shipped weights, each
+    step one HIP-graph replay (fresh right-hand sides every step), with BF16 or FP32
+    activations (FP32 master weights); -> (model, steps/s, last loss, peak HBM GiB,
+    weights source)."""
+    import torch
+    from mlpcg_amd import train as T
+    act = {"bf16": torch.bfloat16, "fp32": torch.float32}[act_name]
+    model, wsrc = load_model(args.weights, args.levels, dev)
+    model.train()
+    opt = torch.optim.Adam(model.parameters(), lr=3e-4, capturable=True)
+    torch.cuda.reset_peak_memory_stats(dev)
+    rhs = [T.white_rhs(sb, 1, gen) for sb in scenes]
+    step = T.GraphedTrainStep(model, scenes, rhs, opt, act_dtype=act)  # one HIP graph per step
+    sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        rhs = [T.white_rhs(sb, 1, gen) for sb in scenes]
+        loss_t = step(rhs)
+        if k % 25 == 0:
+            log(f"[bench] train256 {act_name} step {k} loss {float(loss_t):.4f}")
+    sync()
+    sps = steps / (time.perf_counter() - t0)
+    return model, sps, float(step.loss), torch.cuda.max_memory_allocated(dev) / 2 ** 30, wsrc
+
+
+def train256_rate(args, dev, steps=40):
+    """The 256^3 training step of the train256 workload, BF16 vs FP32 activations
+    (steps/s after the graph capture, no held-out solves)."""
+    import torch
+    scenes = train256_scenes(dev)
+    gen = torch.Generator(device=dev).manual_seed(3)
+    out = {"workload": "4 scenes x 256^3 (40 spheres), 1 RHS each, l=5, Adam from the shipped weights; "
+                       "BF16 vs FP32 activations (FP32 master weights); step captured in a HIP graph",
+           "steps": steps}
+    for name in ("bf16", "fp32"):
+        _, sps, loss, peak, _ = train256_steps(args, dev, scenes, name, steps, gen)
+        out[name] = {"steps_per_s": round(sps, 3), "peak_hbm_GiB": round(peak, 2), "last_loss": loss}
+    out["bf16_vs_fp32"] = round(out["bf16"]["steps_per_s"] / out["fp32"]["steps_per_s"], 3)
+    return out
+
+
 def workload_train256(args, dev):
     """Training at 256^3 (4 scenes, 40 spheres r 4-30, air fraction 20-50 %): BF16
     forward/backward with FP32 master weights vs FP32, stencils recomputed in backward,
@@ -342,28 +393,12 @@ def workload_train256(args, dev):
     from mlpcg_amd import train as T
     from mlpcg_amd.solver import FusedPCG
     from mlpcg_amd import scenes as S
-    scenes = [T.SceneBatch(s, 1, dev) for s in T.make_scenes(256, range(300, 304), "spheres40")]
+    scenes = train256_scenes(dev)
     held = [T.SceneBatch(s, 1, dev) for s in T.make_scenes(256, range(400, 408), "spheres40")]
     out = {"workload": "train 4 x 256^3 (40 spheres), l=5", "steps": args.train_steps}
     gen = torch.Generator(device=dev).manual_seed(3)
-    for act_name, act in (("bf16", torch.bfloat16), ("fp32", torch.float32)):
-        model, wsrc = load_model(args.weights, args.levels, dev)
-        model.train()
-        opt = torch.optim.Adam(model.parameters(), lr=3e-4, capturable=True)
-        torch.cuda.reset_peak_memory_stats(dev)
-        rhs = [T.white_rhs(sb, 1, gen) for sb in scenes]
-        step = T.GraphedTrainStep(model, scenes, rhs, opt, act_dtype=act)  # one HIP graph per step
-        sync()
-        t0 = time.perf_counter()
-        for k in range(args.train_steps):
-            rhs = [T.white_rhs(sb, 1, gen) for sb in scenes]
-            loss_t = step(rhs)
-            if k % 25 == 0:
-                log(f"[bench] train256 {act_name} step {k} loss {float(loss_t):.4f}")
-        sync()
-        sps = args.train_steps / (time.perf_counter() - t0)
-        loss = float(step.loss)
-        peak = torch.cuda.max_memory_allocated(dev) / 2 ** 30
+    for act_name in ("bf16", "fp32"):
+        model, sps, loss, peak, wsrc = train256_steps(args, dev, scenes, act_name, args.train_steps, gen)
         its = []
         model.eval()
         with torch.no_grad():
@@ -541,6 +576,10 @@ def main():
         train["workload"] = "8 scenes x 128^3, 1 RHS each, l=5, FP32, Adam; step captured in a HIP graph"
         log(f"[bench] train: {train['steps_per_s']:.2f} steps/s graphed, "
             f"{train['eager_steps_per_s']:.2f} eager")
+        t256 = train256_rate(args, dev)
+        train["train256"] = t256
+        log(f"[bench] train256: bf16 {t256['bf16']['steps_per_s']:.2f} steps/s, fp32 "
+            f"{t256['fp32']['steps_per_s']:.2f} steps/s ({t256['bf16_vs_fp32']:.2f}x)")
 
     cpu = None
     if not args.skip_cpu:

@@ -57,7 +57,10 @@ int mlpcg_version(void);
 
 /* ---- SM block (replaces torch_extension/sm_block_3d.cpp) -------------------------
  * y[b,v] = sum_p K_p(v) x[b, v+d_p],  K_p(v) = bias[p] + sum_{s,e} weight[p,s,e] image[s, v+e].
- * workspace: mlpcg_sm_block_workspace_size() bytes (stencil weights + gradient partials). */
+ * workspace: mlpcg_sm_block_workspace_size() bytes (stencil weights + gradient partials; for
+ * BF16 also an FP16 image copy and packed weights).  BF16 blocks on grids of >= 2^18 voxels
+ * with n1, n2 % 8 == 0, n3 % 64 == 0 (B = 1) run on the matrix cores: image values must lie
+ * in the FP16 normal range there. */
 size_t mlpcg_sm_block_workspace_size(int C, int B, int n1, int n2, int n3, int dtype);
 /* sm_block_3d.cpp:31  smblock3d.inference(image, x, weights, bias) -> [y] */
 int mlpcg_sm_block_inference(const void* image, const void* x, const void* weight, const void* bias,

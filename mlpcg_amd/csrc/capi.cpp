@@ -62,10 +62,15 @@ const char* mlpcg_last_error(void) { return g_err; }
 int mlpcg_version(void) { return 200; }
 
 // --------------------------------------------------------------------------- SM block
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Workspace: the stencils K [27][V], the weight-gradient partials, and for BF16 an FP16
+// copy of the image plus the packed weights of the matrix-core kernels.
 size_t mlpcg_sm_block_workspace_size(int C, int B, int n1, int n2, int n3, int dtype) {
   const size_t V = (size_t)n1 * n2 * n3;
-  size_t k = (27 * V * elem_size(dtype) + 255) & ~(size_t)255;
-  return k + weight_grad_workspace(dtype, C, B, n1, n2, n3);
+  size_t n = align256(27 * V * elem_size(dtype)) + align256(weight_grad_workspace(dtype, C, B, n1, n2, n3));
+  if (dtype == kBF16 && C >= 1 && C <= 3) n += align256((size_t)C * V * 2) + align256(fused_weights_bytes(C));
+  return n;
 }
 
 static int block_fwd(bool trans, const void* image, const void* x, const void* weight,
@@ -121,12 +126,31 @@ static int block_bwd(bool trans, const void* g, const void* image, const void* x
   MLPCG_REQUIRE(aligned16(g) && aligned16(x) && aligned16(gx) && aligned16(workspace) && aligned16(image),
                 "%s: pointers must be 16-byte aligned", who);
   const size_t V = (size_t)n1 * n2 * n3;
+  const size_t kbytes = align256(27 * V * elem_size(dtype));
+  void* part = (char*)workspace + kbytes;
   // the input gradient is the adjoint apply, both from the forward stencils K (recomputed
   // here, not saved from the forward pass): for the block y = sum_p K_p(v) x(v + d_p) it
   // is the transposed gather sum_p K_p(v - d_p) g(v - d_p); for the transpose block it is
-  // the block apply itself
-  if (launch_stencil_weights(dtype, image, weight, bias, workspace, C, n1, n2, n3, false, STREAM(stream)))
+  // the block apply itself.  BF16 grids that the fused apply takes get K from its first
+  // product (matrix cores, an FP16 image copy), stored instead of applied.
+  if (dtype == kBF16 && V >= kFusedForwardMinVoxels && fused_eligible(C, B, n1, n2, n3)) {
+    char* ext = (char*)part + align256(weight_grad_workspace(dtype, C, B, n1, n2, n3));
+    void* img16 = ext;
+    void* wpack = ext + align256((size_t)C * V * 2);
+    if (launch_cast(kBF16, kF16, image, img16, (long)(C * V), STREAM(stream))) return -1;
+    if (launch_fused_pack(kBF16, weight, bias, C, wpack, STREAM(stream))) return -1;
+    FusedArgs f;
+    memset(&f, 0, sizeof(f));
+    f.img = img16;
+    f.apack = wpack;
+    f.n1 = n1;
+    f.n2 = n2;
+    f.n3 = n3;
+    if (launch_fused_kwrite(C, f, workspace, STREAM(stream))) return -1;
+  } else if (launch_stencil_weights(dtype, image, weight, bias, workspace, C, n1, n2, n3, false,
+                                    STREAM(stream))) {
     return -1;
+  }
   ApplyArgs a;
   memset(&a, 0, sizeof(a));
   a.K = workspace;
@@ -141,7 +165,6 @@ static int block_bwd(bool trans, const void* g, const void* image, const void* x
   } else {
     if (launch_sm_apply_transposed(dtype, a, STREAM(stream))) return -1;
   }
-  void* part = (char*)workspace + ((27 * V * elem_size(dtype) + 255) & ~(size_t)255);
   // block: G_p(v) = sum_b g[b,v] x[b,v+d_p]; transpose: H_p(u) = sum_b x[b,u] g[b,u+d_p]
   const void* av = trans ? x : g;
   const void* cv = trans ? g : x;

@@ -162,9 +162,12 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 }  // namespace
 
-// Two packings of the weights (FP16), then the bias accumulators (FP32).
+// Two packings of the weights (FP16), then the bias accumulators (FP32), then the packing
+// of the K writer (FP16) and its bias accumulators (FP32).
 constexpr size_t fused_apack_bytes(int C) { return (size_t)2 * 2 * nkc(C) * 64 * 8 * sizeof(f16); }
-size_t fused_weights_bytes(int C) { return 2 * fused_apack_bytes(C) + (size_t)2 * 2 * 64 * 4 * sizeof(float); }
+constexpr size_t kBpackBytes = (size_t)2 * 2 * 64 * 4 * sizeof(float);
+constexpr size_t fused_kw_offset(int C) { return 2 * fused_apack_bytes(C) + kBpackBytes; }
+size_t fused_weights_bytes(int C) { return fused_kw_offset(C) + fused_apack_bytes(C) + (size_t)2 * 64 * sizeof(float); }
 constexpr int kBiasGroup = 1;               // lane group whose spare slots carry the bias (slot form)
 constexpr uint32_t kOnesF16 = 0x3c003c00u;  // two FP16 ones
 
@@ -174,7 +177,10 @@ constexpr uint32_t kOnesF16 = 0x3c003c00u;  // two FP16 ones
 // D row 16h + 4 (lane / 16) + i, the element i that lane holds of the result (exact, but
 // 16 registers).  Form 1 puts the bias on the spare slots (1, 6), (1, 7) of channel 0,
 // whose image values the kernel sets to 1, as an FP16 value plus its FP16 remainder (about
-// 22 significant bits), for kernels short of registers.
+// 22 significant bits), for kernels short of registers.  The K writer's packing
+// kw[par][hh][kc][lane][8] is the B operand W^T of the swapped product: element j of lane
+// l is W[tap 16 hh + l % 16][channel kc][tap of slot (l / 16, j)], its bias accumulators
+// kb[hh][lane] = b[16 hh + lane % 16] (taps 27..31 zero).
 __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __restrict__ W,
                                                               const float* __restrict__ b, int C,
                                                               f16* __restrict__ apack) {
@@ -201,6 +207,19 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
     const int rho = 16 * h + 4 * (lane / 16) + i;
     const bool rok = fused_slot_valid(par, row_g(rho), row_j(rho));
     bpack[eb] = rok ? b[fused_tap(par, row_g(rho), row_j(rho))] : 0.f;
+  } else if (e < 2 * total + 2 * 2 * 64 * 4 + total) {
+    f16* kw = (f16*)((char*)apack + fused_kw_offset(C));
+    const int ek = e - 2 * total - 2 * 2 * 64 * 4;
+    const int j = ek % 8, lane = (ek / 8) % 64, kc = (ek / 512) % NK, hh = (ek / (512 * NK)) % 2;
+    const int par = ek / (1024 * NK);
+    const int p = 16 * hh + lane % 16, gk = lane / 16;
+    const bool ok = p < 27 && fused_slot_valid(par, gk, j);
+    kw[ek] = (f16)(ok ? W[((long)p * C + kc) * 27 + fused_tap(par, gk, j)] : 0.f);
+  } else if (e < 2 * total + 2 * 2 * 64 * 4 + total + 2 * 64) {
+    float* kb = (float*)((char*)apack + fused_kw_offset(C) + fused_apack_bytes(C));
+    const int eb = e - 2 * total - 2 * 2 * 64 * 4 - total;
+    const int p = 16 * (eb / 64) + (eb % 64) % 16;
+    kb[eb] = p < 27 ? b[p] : 0.f;
   }
 }
 
@@ -518,6 +537,181 @@ void sm_fused_kernel(FusedArgs a) {
   if (dots) block_partials<2>(red, a.partials);
 }
 
+// K writer: the stencils K_p(v) of the fused apply's tile, stored in the [27][V] BF16 layout
+// of the stored-stencil kernels (the input gradient of a BF16 block needs them).  The first
+// product is taken with its operands swapped, D^T = P^T W^T (same registers: the image
+// words as A, the packing kw as B), so lane (g, n) holds tap 16 hh + n of voxels 4 g + i of
+// a z-parity group; the even and odd groups together give it the 8 consecutive cells
+// 32 h + 8 g .. + 7 of one tap, one 16-byte store.  The image tile is staged as in the apply.
+template <int C, int FI>
+__global__ __launch_bounds__(kThreads) void fused_kwrite_kernel(FusedArgs a, bf16* __restrict__ K) {
+  constexpr int kTile = tile_cells(FI);
+  constexpr int kRows = (FI + 2) * kPj / 4;
+  typedef Mfma<f16>::v8 h8;
+  constexpr int NK = nkc(C);
+  __shared__ __attribute__((aligned(16))) f16 is[C * kTile];
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = lane >> 4, nn = lane & 15;
+  const int tz = a.n3 / kFz, tj = a.n2 / kFj;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x);
+  const int z0 = (int)(t % tz) * kFz;
+  const int j0 = (int)((t / tz) % tj) * kFj;
+  const int i0 = (int)(t / tz / tj) * FI;
+  const long V = (long)a.n1 * a.n2 * a.n3;
+
+  const f16* kw = (const f16*)((const char*)a.apack + fused_kw_offset(C));
+  h8 Wt[2][2][NK];  // [z parity][tap half hh][channel]
+#pragma unroll
+  for (int par = 0; par < 2; ++par)
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+      for (int kc = 0; kc < NK; ++kc)
+        Wt[par][hh][kc] = *(const h8*)(kw + ((((long)par * 2 + hh) * NK + kc) * 64 + lane) * 8);
+  const float* kbias = (const float*)((const char*)kw + fused_apack_bytes(C));
+  const float b0 = kbias[lane], b1 = kbias[64 + lane];
+
+  // image tile (FP16), zero outside the grid: the staging of the apply without x
+  const f16* __restrict__ img = (const f16*)a.img;
+  {
+    constexpr unsigned kOut = 0x80000000u;
+    unsigned zi[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int z = z0 - 1 + (u == 0 ? lane : 64 + (lane & 1));
+      zi[u] = (z >= 0 && z < a.n3) ? (unsigned)z * 2u : kOut;
+    }
+    const long plane = (long)a.n2 * a.n3;
+    const int ipb = (int)(plane * 2), irb = a.n3 * 2;
+    const bool lo_i = i0 > 0, hi_i = i0 + FI < a.n1, lo_j = j0 > 0, hi_j = j0 + kFj < a.n2;
+    const long p0 = (long)(i0 - 1) * plane;
+    auto stage = [&](auto wc) {
+      constexpr int W = decltype(wc)::value;
+#pragma unroll
+      for (int k0 = 0; k0 < kRows; k0 += 13) {
+        constexpr int KB = 13;
+        f16 iv[KB][C][2];
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk) {
+          const int k = k0 + kk;
+          if (k < kRows) {
+            const int r = W + 4 * k, pi = r / kPj, pj = r % kPj;
+            const bool edge_p = pi == 0 || pi == FI + 1, edge_j = pj == 0 || pj == kPj - 1;
+            const bool pok = !edge_p || (pi == 0 ? lo_i : hi_i);
+            const bool rok = pok && (!edge_j || (pj == 0 ? lo_j : hi_j));
+            const long po = edge_p ? (pok ? (long)pi * plane : plane) : (long)pi * plane;
+            const int soff = (j0 + pj - 1);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+              const auto ir = row_rsrc(img + (c * V + p0 + po), rok ? ipb : 0);
+#pragma unroll
+              for (int u = 0; u < 2; ++u) iv[kk][c][u] = rsrc_load(ir, zi[u], soff * irb, (const f16*)nullptr);
+            }
+          }
+        }
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk) {
+          const int r = W + 4 * (k0 + kk);
+          if (k0 + kk < kRows) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + lane] = iv[kk][c][0];
+          }
+        }
+        if (lane < 2) {
+#pragma unroll
+          for (int kk = 0; kk < KB; ++kk) {
+            const int r = W + 4 * (k0 + kk);
+            if (k0 + kk < kRows) {
+#pragma unroll
+              for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + 64 + lane] = iv[kk][c][1];
+            }
+          }
+        }
+      }
+    };
+    switch (w) {
+      case 0: stage(std::integral_constant<int, 0>()); break;
+      case 1: stage(std::integral_constant<int, 1>()); break;
+      case 2: stage(std::integral_constant<int, 2>()); break;
+      default: stage(std::integral_constant<int, 3>()); break;
+    }
+  }
+
+  // the image words of the lane's slots, addressed as in the apply
+  const int o1 = col_off(fused_col(2 * g)) - 1, o2 = col_off(fused_col(2 * g + 1)) - 1;
+  const int o8 = col_off(kSplitCol) - 1;
+  const int r3e = g >= 2 ? 2 : 0, r3o = g >= 2 ? 0 : 2;
+  const int iP = o1 + 2 * nn, iQ = o2 + 2 * nn, i8e = o8 + 2 * nn + r3e, i8o = o8 + 2 * nn + r3o;
+  const int wrow = w * kPz;
+  auto lds_base = [&](int elem) {
+    unsigned b = (unsigned)(elem + wrow) * 2u;
+    asm volatile("" : "+v"(b));
+    return b;
+  };
+  const unsigned bI[6] = {lds_base(iP), lds_base(iP + 2), lds_base(iQ), lds_base(iQ + 2), lds_base(i8e),
+                          lds_base(i8o)};
+  __syncthreads();
+
+  auto row = [&](int q) {
+    const int ii = q >> 1, jj = w + 4 * (q & 1);
+    return ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0;
+  };
+  struct Words {
+    uint32_t im[NK][6];
+  };
+  auto cb_of = [&](int q) { return (((q >> 1) + 1) * kPj + (4 * (q & 1) + 1)) * kPz + 1; };
+  auto load_words = [&](int cbq, int h) {
+    Words r;
+    const int cbyte = 2 * (cbq + 32 * h);
+#pragma unroll
+    for (int kc = 0; kc < NK; ++kc) {
+      const char* ib = (const char*)(is + kc * kTile) + cbyte;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) r.im[kc][k] = *(const uint32_t*)(ib + bI[k]);
+    }
+    return r;
+  };
+  Words cur = load_words(cb_of(0), 0);
+#pragma unroll
+  for (int q = 0; q < FI * 2; ++q) {
+    const long vr = row(q);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int qn = h == 0 ? q : (q + 1 < FI * 2 ? q + 1 : q);
+      const Words nxt = load_words(cb_of(qn), h ^ 1);
+      h8 bv[2][NK];
+#pragma unroll
+      for (int kc = 0; kc < NK; ++kc) {
+        const uint32_t* m = cur.im[kc];
+        bv[0][kc] = __builtin_bit_cast(h8, u32x4{m[0], pack_lo(m[1], m[3]), m[2], m[4]});
+        bv[1][kc] = __builtin_bit_cast(h8, u32x4{m[1], pack_hi(m[0], m[2]), m[3], m[5]});
+      }
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const float bb = hh == 0 ? b0 : b1;
+        f32x4 d[2];
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+          d[par] = f32x4{bb, bb, bb, bb};
+#pragma unroll
+          for (int kc = 0; kc < NK; ++kc) d[par] = Mfma<f16>::run(bv[par][kc], Wt[par][hh][kc], d[par]);
+        }
+        // cells 32 h + 8 g + 2 i (even group) and + 1 (odd group) of tap 16 hh + nn
+        u32x4 o;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const bf16x2 pr = {(bf16)d[0][i], (bf16)d[1][i]};
+          o[i] = __builtin_bit_cast(uint32_t, pr);
+        }
+        const int p = 16 * hh + nn;
+        // plain stores: the halves h = 0, 1 of a tap's 128-byte row segment meet in L2
+        if (p < 27) *(u32x4*)(K + p * V + vr + 32 * h + 8 * g) = o;
+      }
+      cur = nxt;
+    }
+  }
+}
+
 // Tile depth along i: the deepest of 8, 4 planes that keeps the tile's LDS (x and C image
 // channels) within kMaxFusedLds, so at least four workgroups fit a CU, and that still
 // gives kMinFusedBlocks workgroups (about eight per CU) to spread over the chip, else 2 (at
@@ -545,7 +739,8 @@ bool fused_eligible(int C, int B, int n1, int n2, int n3) {
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
   MLPCG_REQUIRE(C >= 1 && C <= 3, "fused_pack: C must be 1, 2 or 3");
   MLPCG_REQUIRE(tw == kBF16, "fused_pack: BF16 plans only (got %d)", tw);
-  const int n = 2 * (2 * 2 * nkc(C) * 64 * 8) + 2 * 2 * 64 * 4;
+  const int total = 2 * 2 * nkc(C) * 64 * 8;
+  const int n = 2 * total + 2 * 2 * 64 * 4 + total + 2 * 64;
   fused_pack_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>((const float*)W, (const float*)b, C,
                                                                        (f16*)wbuf);
   return check_launch("fused_pack");
@@ -584,6 +779,22 @@ static void launch_fused_epi(const FusedArgs& a, hipStream_t st) {
       return launch_fused_t<TX, TY, C, kEpiPrev | kEpiMaskB | kEpiDot0 | kEpiDot1>(a, st);
   }
   launch_fused_t<TX, TY, C, kEpiRuntime>(a, st);
+}
+
+int launch_fused_kwrite(int C, const FusedArgs& a, void* K, hipStream_t st) {
+  MLPCG_REQUIRE(fused_eligible(C, 1, a.n1, a.n2, a.n3), "fused_kwrite: grid (%d,%d,%d) / C=%d not supported",
+                a.n1, a.n2, a.n3, C);
+  const int fi = fused_fi(C, a.n1, a.n2, a.n3);
+  const unsigned nb = fused_blocks(C, a.n1, a.n2, a.n3);
+#define MLPCG_KW(CC)                                                                      \
+  if (C == CC) {                                                                          \
+    if (fi == 8) fused_kwrite_kernel<CC, 8><<<nb, kThreads, 0, st>>>(a, (bf16*)K);         \
+    else if (fi == 4) fused_kwrite_kernel<CC, 4><<<nb, kThreads, 0, st>>>(a, (bf16*)K);    \
+    else fused_kwrite_kernel<CC, 2><<<nb, kThreads, 0, st>>>(a, (bf16*)K);                 \
+  }
+  MLPCG_KW(1) MLPCG_KW(2) MLPCG_KW(3)
+#undef MLPCG_KW
+  return check_launch("fused_kwrite");
 }
 
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& args, hipStream_t st) {

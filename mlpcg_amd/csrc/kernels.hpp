@@ -66,6 +66,9 @@ unsigned fused_blocks(int C, int n1, int n2, int n3);
 bool fused_eligible(int C, int B, int n1, int n2, int n3);
 int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st);
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream_t st);
+// The stencils K of the fused apply's first product (a.img, a.apack, a.n1..n3), stored as
+// BF16 [27][V] (the stored-stencil layout) instead of applied.
+int launch_fused_kwrite(int C, const FusedArgs& a, void* K, hipStream_t st);
 
 // The two coarsest levels of the SmallSM V-cycle in one workgroup (tail.hip): from the
 // output x_in of pre[n-2] (level n-2 grid) to up(c0 x + c1 post[n-1](up(l(pool(x))))) on

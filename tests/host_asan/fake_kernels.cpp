@@ -457,6 +457,18 @@ int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream
   return 0;
 }
 
+int launch_fused_kwrite(int C, const FusedArgs& a, void* K, hipStream_t st) {
+  MLPCG_REQUIRE(fused_eligible(C, 1, a.n1, a.n2, a.n3), "fused_kwrite: not eligible");
+  const size_t V = (size_t)a.n1 * a.n2 * a.n3;
+  fakehip::enqueue(st, [=] {
+    const char* who = "fused_kwrite";
+    RD(a.img, C * V * 2);
+    RD(a.apack, fused_weights_bytes(C));
+    WR(K, 27 * V * 2);
+  });
+  return 0;
+}
+
 int launch_upsample2(int dtype, const UpArgs& a, hipStream_t st) {
   MLPCG_REQUIRE(a.n1 % 2 == 0 && a.n2 % 2 == 0 && a.n3 % 2 == 0, "upsample2: odd grid");
   const size_t V = (size_t)a.n1 * a.n2 * a.n3, s = esz(dtype);

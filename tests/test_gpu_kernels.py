@@ -178,12 +178,14 @@ def test_sm_block_backward_bf16_weight_grad(dev, trans, C, shape):
 
 @pytest.mark.parametrize("trans", [False, True])
 @pytest.mark.parametrize("C,shape", [(1, (8, 8, 64)), (3, (5, 7, 128)), (1, (4, 6, 256)), (2, (2, 3, 512)),
-                                     (1, (3, 5, 8))])
+                                     (1, (3, 5, 8)), (1, (64, 64, 64)), (2, (64, 32, 128)), (3, (128, 128, 256)),
+                                     (1, (256, 128, 256))])
 def test_sm_block_backward_bf16_dx_exact(dev, trans, C, shape):
     """BF16 input gradient (the transposed gather, with z-rows of 1..64 runs per wave, and
-    the forward apply for the transpose block): with exactly representable weights,
-    image and output gradient every stencil and every sum is exact, so dx equals the FP64
-    backward rounded to BF16 bit for bit."""
+    the forward apply for the transpose block; on grids of 2^18 voxels or more the stencils
+    come from the matrix-core K writer, with 2-, 4- and 8-plane tiles): with exactly
+    representable weights, image and output gradient every stencil and every sum is exact,
+    so dx equals the FP64 backward rounded to BF16 bit for bit."""
     from mlpcg_amd import kernels as K
     g = torch.Generator().manual_seed(31 + C)
     span = 2 if C == 1 else 1

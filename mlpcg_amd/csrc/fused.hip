@@ -249,7 +249,7 @@ __host__ __device__ constexpr int fused_waves(int C, int FI, int EPI) {
          : C == 3 ? ((EPI & kEpiDot1) ? 1 : 4)
                   : 1;
 }
-template <typename TX, typename TY, typename TP, int C, int FI, int EPI>
+template <typename TX, typename TY, typename TP, int C, int FI, int EPI, bool PL = false>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(fused_waves(C, FI, EPI), 8)))
 void sm_fused_kernel(FusedArgs a) {
   constexpr int kTile = tile_cells(FI);
@@ -456,6 +456,7 @@ void sm_fused_kernel(FusedArgs a) {
   };
   Words cur = load_words(cb_of(0), 0);
   long vr = row(0);
+  float pp[PL ? FI * 2 : 1];  // PL: the output's z-pair sums of each chunk
   uint8_t ka = LA ? lane_at(mA + vr, mob) : uint8_t(1), kb = LB ? lane_at(mB + vr, mob) : uint8_t(1);
   TP pv = LP ? lane_at(pP + vr, mop) : TP(0.f);
   float da = (LD0 && dots) ? lane_at(d0p + vr, mof) : 0.f;
@@ -523,6 +524,11 @@ void sm_fused_kernel(FusedArgs a) {
     if (has_b && kb == 0) val = 0.f;
     const TY yv = (TY)val;
     lane_store(y + vr, moy, yv);
+    if constexpr (PL) {
+      // lanes l and l ^ 16 hold the even and odd voxel of a z pair
+      const float yr = (float)yv;
+      pp[q] = yr + __shfl_xor(yr, 16);
+    }
     if (dots) {
       red[0] += (double)(float)yv * (double)da;
       if (LD1) red[1] += (double)(float)yv * (double)db;
@@ -535,6 +541,30 @@ void sm_fused_kernel(FusedArgs a) {
     db = db_n;
   }
   if (dots) block_partials<2>(red, a.partials);
+  if constexpr (PL) {
+    // the 2x2x2 means of the stored output (the next level's input), summed in the order of
+    // avgpool2_kernel: z pair, then (plane 2I, rows 2J, 2J+1), then plane 2I+1's rows.  Rows
+    // 2J and 2J+1 belong to different waves, so the pair sums meet in LDS (the tiles' space).
+    __syncthreads();
+    float* P = (float*)xs;  // [FI][8 rows][32 coarse z]
+    static_assert(FI * 8 * 32 * 4 <= kTile * 2, "pair sums fit the x tile");
+    if (!(g & 1)) {
+#pragma unroll
+      for (int q = 0; q < FI * 2; ++q) P[((q >> 1) * 8 + w + 4 * (q & 1)) * 32 + 16 * (g >> 1) + nn] = pp[q];
+    }
+    __syncthreads();
+    TY* __restrict__ out = (TY*)a.pool_out;
+    const int c2 = a.n2 >> 1, c3 = a.n3 >> 1;
+    for (int e = threadIdx.x; e < (FI / 2) * 4 * 32; e += kThreads) {
+      const int kc = e & 31, J = (e >> 5) & 3, I = e >> 7;
+      const float* p0 = P + (2 * I * 8 + 2 * J) * 32 + kc;
+      float sum = p0[0];
+      sum += p0[32];
+      sum += p0[8 * 32];
+      sum += p0[8 * 32 + 32];
+      out[((long)(i0 / 2 + I) * c2 + (j0 / 2 + J)) * c3 + z0 / 2 + kc] = (TY)(sum * 0.125f);
+    }
+  }
 }
 
 // K writer: the stencils K_p(v) of the fused apply's tile, stored in the [27][V] BF16 layout
@@ -746,15 +776,15 @@ int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, h
   return check_launch("fused_pack");
 }
 
-template <typename TX, typename TY, int C, int EPI>
+template <typename TX, typename TY, int C, int EPI, bool PL = false>
 static void launch_fused_t(const FusedArgs& a, hipStream_t st) {
   const unsigned nb = fused_blocks(C, a.n1, a.n2, a.n3);
   const int fi = fused_fi(C, a.n1, a.n2, a.n3);
   if constexpr ((C + 1) * tile_cells(8) * 2 <= kMaxFusedLds) {
-    if (fi == 8) return sm_fused_kernel<TX, TY, bf16, C, 8, EPI><<<nb, kThreads, 0, st>>>(a);
+    if (fi == 8) return sm_fused_kernel<TX, TY, bf16, C, 8, EPI, PL><<<nb, kThreads, 0, st>>>(a);
   }
-  if (fi == 4) sm_fused_kernel<TX, TY, bf16, C, 4, EPI><<<nb, kThreads, 0, st>>>(a);
-  else sm_fused_kernel<TX, TY, bf16, C, 2, EPI><<<nb, kThreads, 0, st>>>(a);
+  if (fi == 4) sm_fused_kernel<TX, TY, bf16, C, 4, EPI, PL><<<nb, kThreads, 0, st>>>(a);
+  else sm_fused_kernel<TX, TY, bf16, C, 2, EPI, PL><<<nb, kThreads, 0, st>>>(a);
 }
 
 // The epilogues of a preconditioner application get compile-time kernels: the pre blocks
@@ -767,7 +797,7 @@ static void launch_fused_epi(const FusedArgs& a, hipStream_t st) {
   const int epi = (a.prev ? kEpiPrev : 0) | (a.mask_a ? kEpiMaskA : 0) | (a.mask_b ? kEpiMaskB : 0) |
                   (a.partials ? kEpiDot0 : 0) | (a.partials && a.dot1 ? kEpiDot1 : 0);
   if constexpr (sizeof(TY) == 2) {
-    if (epi == 0) return launch_fused_t<TX, TY, C, 0>(a, st);
+    if (epi == 0) return a.pool_out ? launch_fused_t<TX, TY, C, 0, true>(a, st) : launch_fused_t<TX, TY, C, 0>(a, st);
     if constexpr (sizeof(TX) == 2)
       if (epi == kEpiPrev) return launch_fused_t<TX, TY, C, kEpiPrev>(a, st);
   } else {
@@ -803,6 +833,8 @@ int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& args, hipStr
                 "sm_fused: grid (%d,%d,%d) / C=%d not supported", args.n1, args.n2, args.n3, C);
   MLPCG_REQUIRE(!args.dot1 || args.dot0, "sm_fused: dot1 needs dot0");
   MLPCG_REQUIRE(!args.partials || args.dot0, "sm_fused: partials need dot0");
+  MLPCG_REQUIRE(!args.pool_out || (ty == kBF16 && !args.prev && !args.mask_a && !args.mask_b && !args.partials),
+                "sm_fused: the pooled output needs a BF16 output and no epilogue operands");
   const FusedArgs& a = args;
 #define MLPCG_FU(TXX, TYY)                                 \
   do {                                                     \

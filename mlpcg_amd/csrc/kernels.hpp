@@ -58,6 +58,9 @@ struct FusedArgs {
   const float* dot0;
   const float* dot1;
   double* partials;
+  // optional (BF16 output, no epilogue operands): also write the 2x2x2 means of y, the
+  // next level's input, as avgpool2 computes them (same sums in the same order)
+  void* pool_out;
   int n1, n2, n3;
 };
 

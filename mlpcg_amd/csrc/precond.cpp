@@ -199,9 +199,10 @@ int Precond::update(const void* image, const void* const* weights, const void* c
 int Precond::run_block(int k, int level, const void* x, int tx, void* y, int ty, const void* prev,
                        const void* c0, const void* c1, const uint8_t* ma, const uint8_t* mb,
                        const double* sc, const int* done, hipStream_t st, const float* dot0,
-                       const float* dot1, double* partials) {
+                       const float* dot1, double* partials, void* pool_out) {
   if (blocks[k].fused) {
     FusedArgs f;
+    f.pool_out = pool_out;
     f.dot0 = dot0;
     f.dot1 = dot1;
     f.partials = partials;
@@ -292,12 +293,20 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
   } while (0)
 
   // ---- downward half (identical for every kind): x_0 = B0(r), x_i = Bi(pool x_{i-1}) ----
-  TRY(apply_blk(0, 0, r_in, vdt, xd[0], dtype, nullptr, nullptr, nullptr, mask ? maskimg : nullptr,
-                nullptr, in_scale));
   const int ndown = tail ? n - 1 : n;  // with the tail, levels n-1 and n run in one kernel
+  // x_i is pooled into tmp[i+1] for the next block (the tail pools its input itself); a
+  // fused block without epilogue operands (not level 0 under the model mask) writes that
+  // mean in its epilogue, so the pool kernel is left out
+  auto pool_into = [&](int i) -> void* {
+    const bool pooled = (i + 1 < ndown) || (!tail && i == n - 1);
+    return (pooled && blocks[i].fused && !(i == 0 && mask)) ? tmp[i + 1] : nullptr;
+  };
+  TRY(run_block(0, 0, r_in, vdt, xd[0], dtype, nullptr, nullptr, nullptr, mask ? maskimg : nullptr, nullptr,
+                in_scale, done, st, nullptr, nullptr, nullptr, pool_into(0)));
   for (int i = 1; i < ndown; ++i) {
-    TRY(pool(xd[i - 1], tmp[i], i - 1));
-    TRY(apply_blk(i, i, tmp[i], dtype, xd[i], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+    if (!pool_into(i - 1)) TRY(pool(xd[i - 1], tmp[i], i - 1));
+    TRY(run_block(i, i, tmp[i], dtype, xd[i], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, done,
+                  st, nullptr, nullptr, nullptr, pool_into(i)));
   }
   const int kl = (kind == kSmallSPD) ? n : 2 * n;  // index of l
   if (tail) {
@@ -316,7 +325,7 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
     t.f3 = dims[n - 1][2];
     TRY(launch_coarse_tail(dtype, t, st));
   } else {
-    TRY(pool(xd[n - 1], tmp[n], n - 1));
+    if (!pool_into(n - 1)) TRY(pool(xd[n - 1], tmp[n], n - 1));
     TRY(apply_blk(kl, n, tmp[n], dtype, xd[n], dtype, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
   }
 

@@ -62,7 +62,7 @@ struct Precond {
   int run_block(int k, int level, const void* x, int tx, void* y, int ty, const void* prev,
                 const void* c0, const void* c1, const uint8_t* ma, const uint8_t* mb,
                 const double* sc, const int* done, hipStream_t st, const float* dot0 = nullptr,
-                const float* dot1 = nullptr, double* partials = nullptr);
+                const float* dot1 = nullptr, double* partials = nullptr, void* pool_out = nullptr);
   ~Precond();
 };
 

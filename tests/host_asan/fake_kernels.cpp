@@ -440,6 +440,7 @@ int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream
     RD(a.apack, fused_weights_bytes(C));
     RD(a.x, V * esz(tx));
     WR(a.y, V * esz(ty));
+    if (a.pool_out) WR(a.pool_out, V / 8 * esz(ty));
     if (a.prev) {
       RD(a.prev, V * 2);
       RD(a.c0, 4);

@@ -478,39 +478,39 @@ __global__ __launch_bounds__(kThreads) void upsample2_kernel(UpArgs a) {
   if (a.prev) load_vec<VZ>((const TP*)a.prev + off, pv);
   A out[VZ];
   if constexpr (VZ >= 2) {
-    // coarse z segment [k0/2 - 1, k0/2 + VZ/2] of the four rows, indices clamped to the
-    // grid: with the clamp the interior weights reproduce the edge cases exactly
-    // (f = 0 -> x[0]; f = 2m3-1 -> x[m3-1]), so the taps below are compile-time
-    // The two end cells come from the neighbouring lanes (consecutive lanes hold
-    // consecutive segments of a row) by DPP shifts; lanes 0 / 63 load them, and at the
-    // row ends the clamp makes them the segment's own end values.
+    // Separable: the four coarse rows are combined per coarse cell first (u = the (i, j)
+    // interpolation), then u is interpolated along z.  u covers the coarse z segment
+    // [k0/2 - 1, k0/2 + VZ/2], indices clamped to the grid: with the clamp the interior
+    // weights reproduce the edge cases exactly (f = 0 -> u[0]; f = 2m3-1 -> u[m3-1]), so
+    // the z taps below are compile-time.  The two end cells of u come from the
+    // neighbouring lanes (consecutive lanes hold consecutive segments of a row) by DPP
+    // shifts; lanes 0 / 63 form them from loads, and at the row ends the clamp makes them
+    // the segment's own end values.
     constexpr int NS = VZ / 2 + 2;
     const int kc = c.k0 >> 1;
     const int klo = max(kc - 1, 0), khi = min(kc + VZ / 2, m3 - 1);
     const int lane = threadIdx.x & 63;
     const bool row_start = c.k0 == 0, row_end = c.k0 + VZ >= a.n3;
-    A seg[4][NS];
+    A seg[4][VZ / 2];
     const TI* rows[4] = {r00, r01, r10, r11};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) load_vec<VZ / 2>(rows[q] + kc, &seg[q][1]);
+    for (int q = 0; q < 4; ++q) load_vec<VZ / 2>(rows[q] + kc, seg[q]);
+    auto comb = [&](A x00, A x01, A x10, A x11) { return hi0 * (hj0 * x00 + hj1 * x01) + hi1 * (hj0 * x10 + hj1 * x11); };
+    A u[NS];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const A lo = wave_from_prev(seg[q][VZ / 2]), hi = wave_from_next(seg[q][1]);
-      seg[q][0] = row_start ? seg[q][1] : (lane == 0 ? (A)rows[q][klo] : lo);
-      seg[q][NS - 1] = row_end ? seg[q][VZ / 2] : (lane == 63 ? (A)rows[q][khi] : hi);
-    }
+    for (int s = 0; s < VZ / 2; ++s) u[s + 1] = comb(seg[0][s], seg[1][s], seg[2][s], seg[3][s]);
+    const A lo = wave_from_prev(u[VZ / 2]), hi = wave_from_next(u[1]);
+    u[0] = row_start ? u[1]
+                     : (lane == 0 ? comb((A)r00[klo], (A)r01[klo], (A)r10[klo], (A)r11[klo]) : lo);
+    u[NS - 1] = row_end ? u[VZ / 2]
+                        : (lane == 63 ? comb((A)r00[khi], (A)r01[khi], (A)r10[khi], (A)r11[khi]) : hi);
 #pragma unroll
     for (int t = 0; t < VZ; ++t) {
-      A z[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (a.nearest) z[q] = seg[q][t / 2 + 1];
-        else if ((t & 1) == 0) z[q] = A(0.25) * seg[q][t / 2] + A(0.75) * seg[q][t / 2 + 1];
-        else z[q] = A(0.75) * seg[q][t / 2 + 1] + A(0.25) * seg[q][t / 2 + 2];
-      }
-      const A v0 = hj0 * z[0] + hj1 * z[1];
-      const A v1 = hj0 * z[2] + hj1 * z[3];
-      out[t] = (hi0 * v0 + hi1 * v1) * (A)a.scale;
+      A z;
+      if (a.nearest) z = u[t / 2 + 1];
+      else if ((t & 1) == 0) z = A(0.25) * u[t / 2] + A(0.75) * u[t / 2 + 1];
+      else z = A(0.75) * u[t / 2 + 1] + A(0.25) * u[t / 2 + 2];
+      out[t] = z * (A)a.scale;
     }
   } else {
 #pragma unroll

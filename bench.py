@@ -18,7 +18,8 @@ profiles/sm_apply_bf16_level0_pillars256_hbm_traffic.txt for the 256^3 workload 
 (null otherwise).
 
 three_channel: the same scene solved with the three-channel model (image channels
-[air, fluid, non-solid], mlpcg_amd/weights/sm_l5_c3.pt) by the BF16 fused plan, the BF16
+[air, fluid, non-solid], mlpcg_amd/weights/sm_l5_c3.pt) by the BF16 fused plan, the same
+with the non-solid channel folded into the other two in the fused blocks, the BF16
 stored-stencil plan and the FP32 plan (--skip-c3 leaves it out).
 
 train: Adam steps/s of the graph-captured training step on 8 x 128^3 scenes (FP32), and
@@ -124,16 +125,20 @@ def load_model(path, levels, dev, num_imgs=1):
 
 def three_channel_solves(args, sc, solver, b, dev):
     """The same scene and RHS with a three-channel model ([air, fluid, non-solid] images,
-    the reference's own models): BF16 with the fused matrix-core apply, BF16 with stored
-    stencils, FP32; npsd to the same tolerance."""
+    the reference's own models): BF16 with the fused matrix-core apply, the same with the
+    non-solid channel folded into the other two, BF16 with stored stencils, FP32; npsd to
+    the same tolerance."""
     import torch
     from mlpcg_amd import scenes as S
     model, wsrc = load_model(C3_WEIGHTS, args.levels, dev, num_imgs=3)
     image = torch.as_tensor(S.image_channels(sc.flags, 3), device=dev)
     out = {"weights": wsrc}
-    for name, dtype, fused in (("bf16_fused", torch.bfloat16, 1), ("bf16_stored", torch.bfloat16, 0),
-                               ("fp32", torch.float32, 0)):
-        plan = model.make_preconditioner(image, dtype, fused_mode=fused)
+    # bf16_fused_folded: the fused blocks take [air, fluid] with the non-solid channel's
+    # weights added to theirs (non-solid = air + fluid on these images)
+    for name, dtype, fused, fold in (("bf16_fused", torch.bfloat16, 1, False),
+                                     ("bf16_fused_folded", torch.bfloat16, 1, True),
+                                     ("bf16_stored", torch.bfloat16, 0, False), ("fp32", torch.float32, 0, False)):
+        plan = model.make_preconditioner(image, dtype, fused_mode=fused, fold_third=fold)
 
         def step():
             plan.update(image)

@@ -203,10 +203,13 @@ void mlpcg_precond_destroy(mlpcg_precond* P);
 /* mlpcg_precond_create with the plan's execution policies given explicitly (create uses
  * fused_mode 1 and coarse_tail 1).  No counterpart in the reference, whose SM blocks
  * always evaluate the weighted stencil on the fly:
- *   fused_mode: BF16 blocks on grids with n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 2, not
+ *   fused_mode: BF16 blocks on grids with n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 3, not
  *     transposed) recompute K on the matrix cores inside the apply instead of storing it.
  *     0: never (K stored like every other block); 1: on such grids of at least 2^18
- *     voxels (smaller ones are faster with stored K); 2: on every such grid.
+ *     voxels (smaller ones are faster with stored K); 2: on every such grid.  Plus 4
+ *     (C = 3 only): the image's third channel is the sum of the first two (the
+ *     [air, fluid, non-solid] images), so the fused blocks take two channels with the third
+ *     channel's weights added to the others' -- the caller vouches for the relation.
  *   coarse_tail: SmallSM plans (swap off, FP32/FP16/BF16) run their two coarsest levels as
  *     one workgroup kernel with the level vectors in LDS (1) instead of seven small kernels
  *     (0), when level n-1 has even dims and at most 4096 voxels (16-bit storage,

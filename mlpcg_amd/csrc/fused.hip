@@ -183,7 +183,11 @@ constexpr uint32_t kOnesF16 = 0x3c003c00u;  // two FP16 ones
 // kb[hh][lane] = b[16 hh + lane % 16] (taps 27..31 zero).
 __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __restrict__ W,
                                                               const float* __restrict__ b, int C,
-                                                              f16* __restrict__ apack) {
+                                                              int fold, f16* __restrict__ apack) {
+  // weight of tap p, packed channel kc, input tap q (fold: C = 2 packed channels from three)
+  auto wt = [&](int p, int kc, int q) {
+    return fold ? W[((long)p * 3 + kc) * 27 + q] + W[((long)p * 3 + 2) * 27 + q] : W[((long)p * C + kc) * 27 + q];
+  };
   const int NK = nkc(C);
   const int total = 2 * 2 * NK * 64 * 8;
   // one element per thread over the grid: the two packings, then the bias accumulators
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
     const int rho = 16 * h + lane % 16, gk = lane / 16;
     const bool rok = fused_slot_valid(par, row_g(rho), row_j(rho));
     const int p = fused_tap(par, row_g(rho), row_j(rho)), q = fused_tap(par, gk, j);
-    float v = rok && fused_slot_valid(par, gk, j) ? W[((long)p * C + kc) * 27 + q] : 0.f;
+    float v = rok && fused_slot_valid(par, gk, j) ? wt(p, kc, q) : 0.f;
     if (form == 1 && rok && kc == 0 && gk == kBiasGroup && j >= 6) {
       const float hi = (float)(f16)b[p];
       v = j == 6 ? hi : b[p] - hi;
@@ -214,7 +218,7 @@ __global__ __launch_bounds__(kThreads) void fused_pack_kernel(const float* __res
     const int par = ek / (1024 * NK);
     const int p = 16 * hh + lane % 16, gk = lane / 16;
     const bool ok = p < 27 && fused_slot_valid(par, gk, j);
-    kw[ek] = (f16)(ok ? W[((long)p * C + kc) * 27 + fused_tap(par, gk, j)] : 0.f);
+    kw[ek] = (f16)(ok ? wt(p, kc, fused_tap(par, gk, j)) : 0.f);
   } else if (e < 2 * total + 2 * 2 * 64 * 4 + total + 2 * 64) {
     float* kb = (float*)((char*)apack + fused_kw_offset(C) + fused_apack_bytes(C));
     const int eb = e - 2 * total - 2 * 2 * 64 * 4 - total;
@@ -766,13 +770,15 @@ bool fused_eligible(int C, int B, int n1, int n2, int n3) {
          (long)n2 * n3 * 4 < (1L << 31);
 }
 
-int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
+int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st, bool fold) {
   MLPCG_REQUIRE(C >= 1 && C <= 3, "fused_pack: C must be 1, 2 or 3");
+  MLPCG_REQUIRE(!fold || C == 3, "fused_pack: folding needs C = 3");
+  const int Cp = fold ? 2 : C;  // packed channels
   MLPCG_REQUIRE(tw == kBF16, "fused_pack: BF16 plans only (got %d)", tw);
-  const int total = 2 * 2 * nkc(C) * 64 * 8;
+  const int total = 2 * 2 * nkc(Cp) * 64 * 8;
   const int n = 2 * total + 2 * 2 * 64 * 4 + total + 2 * 64;
-  fused_pack_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>((const float*)W, (const float*)b, C,
-                                                                       (f16*)wbuf);
+  fused_pack_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>((const float*)W, (const float*)b, Cp,
+                                                                       fold ? 1 : 0, (f16*)wbuf);
   return check_launch("fused_pack");
 }
 

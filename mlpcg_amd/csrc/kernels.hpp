@@ -67,7 +67,8 @@ struct FusedArgs {
 size_t fused_weights_bytes(int C);
 unsigned fused_blocks(int C, int n1, int n2, int n3);
 bool fused_eligible(int C, int B, int n1, int n2, int n3);
-int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st);
+// W [27][C][27] (FP32); fold (C = 3): pack two channels, W[p][c] + W[p][2] for c = 0, 1
+int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st, bool fold = false);
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream_t st);
 // The stencils K of the fused apply's first product (a.img, a.apack, a.n1..n3), stored as
 // BF16 [27][V] (the stored-stencil layout) instead of applied.

@@ -61,7 +61,10 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
                 "precond_create: grid (%d,%d,%d) must be divisible by 2^levels = %d", n1, n2, n3, div);
   MLPCG_REQUIRE(!mask || C >= 2, "precond_create: mask needs the fluid image channel (C >= 2)");
   MLPCG_REQUIRE(kind == kSmallSM || (!mask && !swap), "precond_create: mask/swap only for SmallSM");
-  MLPCG_REQUIRE(fused_mode >= 0 && fused_mode <= 2, "precond_create: fused mode must be 0, 1 or 2");
+  const int fold = (fused_mode & kFuseFoldThird) ? 1 : 0;
+  fused_mode &= ~kFuseFoldThird;
+  MLPCG_REQUIRE(fused_mode >= 0 && fused_mode <= 2, "precond_create: fused mode must be 0, 1 or 2 (+4)");
+  MLPCG_REQUIRE(!fold || C == 3, "precond_create: folding the third image channel needs C = 3");
   Precond* P = new Precond();
   P->gen = g_next_gen.fetch_add(1);
   P->kind = kind;
@@ -120,15 +123,16 @@ int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3
     P->nw = n + 1;
   }
   const bool half = (dtype == kBF16);
+  P->cf = fold ? 2 : C;
   for (Precond::Blk& b : P->blocks) {
     const int* d = P->dims[b.level];
     void* K = nullptr;
     void* wp = nullptr;
     const bool big = P->V[b.level] >= kFusedMinVoxels || fused_mode == 2;
-    b.fused = (half && !b.trans && fused_mode && big && fused_eligible(C, 1, d[0], d[1], d[2])) ? 1 : 0;
+    b.fused = (half && !b.trans && fused_mode && big && fused_eligible(P->cf, 1, d[0], d[1], d[2])) ? 1 : 0;
     if (b.fused) {
-      TRY(palloc(P, &wp, fused_weights_bytes(C)));
-      if (!P->imgw[b.level]) TRY(palloc(P, &P->imgw[b.level], (size_t)C * P->V[b.level] * es));
+      TRY(palloc(P, &wp, fused_weights_bytes(P->cf)));
+      if (!P->imgw[b.level]) TRY(palloc(P, &P->imgw[b.level], (size_t)P->cf * P->V[b.level] * es));
     } else {
       TRY(palloc(P, &K, (size_t)27 * P->V[b.level] * es));
     }
@@ -169,12 +173,12 @@ int Precond::update(const void* image, const void* const* weights, const void* c
                         nullptr, st));
   if (mask) TRY(launch_nonzero_mask(acc, (const char*)img[0] + (size_t)V[0] * as, maskimg, V[0], st));
   for (int l = 0; l <= n; ++l)
-    if (imgw[l]) TRY(launch_cast(acc, kF16, img[l], imgw[l], (long)C * V[l], st));  // FP16 MFMA input
+    if (imgw[l]) TRY(launch_cast(acc, kF16, img[l], imgw[l], (long)cf * V[l], st));  // FP16 MFMA input
   for (size_t k = 0; k < blocks.size(); ++k) {
     const Blk& b = blocks[k];
     const int* d = dims[b.level];
     if (b.fused)
-      TRY(launch_fused_pack(dtype, weights[b.w], biases[b.w], C, wpack[k], st));
+      TRY(launch_fused_pack(dtype, weights[b.w], biases[b.w], C, wpack[k], st, cf < C));
     else
       TRY(launch_stencil_weights2(acc, dtype, img[b.level], weights[b.w], biases[b.w], K[k], C, d[0],
                                   d[1], d[2], b.trans != 0, st));
@@ -220,7 +224,7 @@ int Precond::run_block(int k, int level, const void* x, int tx, void* y, int ty,
     f.n1 = dims[level][0];
     f.n2 = dims[level][1];
     f.n3 = dims[level][2];
-    return launch_sm_fused(dtype, tx, ty, C, f, st);
+    return launch_sm_fused(dtype, tx, ty, cf, f, st);
   }
   ApplyArgs a;
   a.K = K[k];
@@ -341,11 +345,11 @@ int Precond::apply(const float* r_in, float* z_out, const double* in_scale, cons
       if (!swap) {
         if (!(tail && i == n - 1)) TRY(up(cur, tmp[lvl], lvl, 1.0, nullptr, nullptr, nullptr));
         if (last && dot0 && partials && dot_blocks && blocks[n].fused &&
-            2L * fused_blocks(C, dims[0][0], dims[0][1], dims[0][2]) <= reduction_capacity()) {
+            2L * fused_blocks(cf, dims[0][0], dims[0][1], dims[0][2]) <= reduction_capacity()) {
           // the solver's dot products on z ride in the epilogue of the last block
           TRY(run_block(n, 0, tmp[0], dtype, y, ty, xd[0], c0p(0, 0), c1p(0, 0), ma, mb, nullptr, done, st,
                         dot0, dot1, partials));
-          *dot_blocks = fused_blocks(C, dims[0][0], dims[0][1], dims[0][2]);
+          *dot_blocks = fused_blocks(cf, dims[0][0], dims[0][1], dims[0][2]);
           cur = xo[lvl];
           continue;
         }

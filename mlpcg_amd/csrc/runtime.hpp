@@ -40,7 +40,9 @@ struct Precond {
   };
   std::vector<Blk> blocks;          // one per entry of K (nullptr for fused blocks)
   std::vector<void*> wpack;         // fused blocks: packed MFMA weights + bias init
-  void* imgw[kMaxLevels + 1] = {};  // FP16 image of levels with fused blocks
+  void* imgw[kMaxLevels + 1] = {};  // FP16 image of levels with fused blocks (cf channels)
+  int cf = 1;                       // image channels of the fused blocks: C, or 2 when the
+                                    // third channel is folded into the first two
   int tail = 0;                     // SmallSM: levels n-1 and n in one workgroup kernel
   int nw = 0;                       // number of block weights before c0/c1
   void* lin_ws = nullptr;           // SMLinear reduction workspace
@@ -68,9 +70,13 @@ struct Precond {
 
 // Per-plan policies.  fused_mode: eligible BF16 blocks recompute their stencils on the
 // matrix cores inside the apply -- 0 never, 1 (default) on grids of at least
-// kFusedMinVoxels voxels, 2 on every eligible grid.  coarse_tail: the two coarsest SmallSM
+// kFusedMinVoxels voxels, 2 on every eligible grid; plus kFuseFoldThird: for three-channel
+// images whose third channel is the sum of the first two (the [air, fluid, non-solid]
+// images), the fused blocks take two channels with the third channel's weights added to
+// the others' (the caller vouches for the relation).  coarse_tail: the two coarsest SmallSM
 // levels run as one workgroup kernel when they fit (default on).
 constexpr int kDefaultFusedMode = 1;
+constexpr int kFuseFoldThird = 4;
 constexpr int kDefaultCoarseTail = 1;
 
 int precond_create(Precond** out, int kind, int n, int C, int n1, int n2, int n3, int dtype,

@@ -42,14 +42,18 @@ class Preconditioner:
     (torch.float32 / bfloat16 / float16; float64 for checking).
 
     Execution policies of this plan (no effect on the result beyond FP32 summation order):
-    fused_mode -- BF16 blocks on grids with n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 2)
+    fused_mode -- BF16 blocks on grids with n1, n2 % 8 == 0 and n3 % 64 == 0 (C <= 3)
       recompute their stencils on the matrix cores inside the apply instead of storing
       them: 0 never, 1 (default) on such grids of at least 2^18 voxels, 2 on every such grid;
+    fold_third -- three-channel images whose third channel is the sum of the first two (the
+      [air, fluid, non-solid] images of the reference's models): the fused blocks take two
+      channels, the third channel's weights added to the others' (equal in exact arithmetic;
+      the caller vouches for the relation, which the plan does not check);
     coarse_tail -- the two coarsest SmallSM levels run as one workgroup kernel when level
       n-1 is small enough (even dims; at most 4096 voxels with 16-bit storage and
       n3 % 4 == 0, else 2048)."""
 
-    def __init__(self, model, image, dtype=torch.float32, fused_mode=1, coarse_tail=True):
+    def __init__(self, model, image, dtype=torch.float32, fused_mode=1, coarse_tail=True, fold_third=False):
         kind = _KINDS.get(type(model).__name__)
         if kind is None:
             raise MlpcgError(f"no native plan for {type(model).__name__}")
@@ -81,7 +85,8 @@ class Preconditioner:
                                           code_of(dtype), int(mode == "nearest"),
                                           int(bool(getattr(model, "mask", False))),
                                           int(bool(getattr(model, "swap_ord", False))),
-                                          int(fused_mode), int(bool(coarse_tail)), ptr(self._img),
+                                          int(fused_mode) | (4 if fold_third else 0),
+                                          int(bool(coarse_tail)), ptr(self._img),
                                           warr, barr, stream_ptr(self.device)), "precond_create")
         self._h = handle
         self._level_numel = [(n1 >> l) * (n2 >> l) * (n3 >> l) for l in range(self.n + 1)]

@@ -400,11 +400,13 @@ class SmallSMModelDn3D(BaseModel):
         lin = list(self.c0) + list(self.c1)
         return [m.weight for m in blocks + lin], [m.bias for m in blocks + lin]
 
-    def make_preconditioner(self, image, dtype=torch.float32, fused_mode=1, coarse_tail=True):
+    def make_preconditioner(self, image, dtype=torch.float32, fused_mode=1, coarse_tail=True, fold_third=False):
         """Native plan with K, the image pyramid and c0/c1 precomputed for `image`;
-        fused_mode / coarse_tail are the plan's execution policies (precond.Preconditioner)."""
+        fused_mode / coarse_tail / fold_third are the plan's execution policies
+        (precond.Preconditioner)."""
         from .precond import Preconditioner
-        return Preconditioner(self, image, dtype, fused_mode=fused_mode, coarse_tail=coarse_tail)
+        return Preconditioner(self, image, dtype, fused_mode=fused_mode, coarse_tail=coarse_tail,
+                              fold_third=fold_third)
 
 
 ####################################

@@ -633,13 +633,13 @@ int launch_cast(int tin, int tout, const void* in, void* out, long n, hipStream_
   return 0;
 }
 
-int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st) {
-  MLPCG_REQUIRE(tw == kBF16 && C >= 1 && C <= 3, "fused_pack: not eligible");
+int launch_fused_pack(int tw, const void* W, const void* b, int C, void* wbuf, hipStream_t st, bool fold) {
+  MLPCG_REQUIRE(tw == kBF16 && C >= 1 && C <= 3 && (!fold || C == 3), "fused_pack: not eligible");
   fakehip::enqueue(st, [=] {
     const char* who = "fused_pack";
     RD(W, 27 * 27 * C * 4);
     RD(b, 27 * 4);
-    WR(wbuf, fused_weights_bytes(C));
+    WR(wbuf, fused_weights_bytes(fold ? 2 : C));
   });
   return 0;
 }

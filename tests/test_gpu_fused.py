@@ -96,6 +96,60 @@ def test_fused_matches_stored_stencils(dev, C, mask, n, shape):
     assert torch.all(zf2[om == 0] == 0)
 
 
+@pytest.mark.parametrize("n,shape", [(2, (32, 16, 128)), (1, (64, 64, 128))])
+def test_fused_fold_third_channel_exact(dev, n, shape):
+    """Three-channel images [a, f, a + f] (the [air, fluid, non-solid] structure): plans
+    whose fused blocks fold the third channel into the first two compute, with exactly
+    representable weights and images, the same stencils as the three-channel plans, so
+    every block and the whole application agree bit for bit."""
+    m = _exact_model(n, 3, False, seed=21)
+    g = torch.Generator().manual_seed(22)
+    f = 2 ** n
+    cs = tuple(d // f for d in shape)
+    a = torch.randint(0, 3, cs, generator=g).float() / 4
+    fl = torch.randint(0, 2, cs, generator=g).float() / 4
+    img = torch.stack([a, fl, a + fl])
+    for ax in (1, 2, 3):
+        img = img.repeat_interleave(f, dim=ax)
+    img = img.contiguous().cuda()
+    r = torch.randint(-4, 5, shape, generator=torch.Generator().manual_seed(23)).float().cuda()
+    p3 = m.make_preconditioner(img, torch.bfloat16, fused_mode=2)
+    p2 = m.make_preconditioner(img, torch.bfloat16, fused_mode=2, fold_third=True)
+    assert p2.fused_blocks == p3.fused_blocks > 0
+    assert p2.nbytes < p3.nbytes  # two-channel FP16 image copies
+    sc = torch.tensor(0.5, dtype=torch.float64, device=dev)
+    assert torch.equal(p2.apply_block(0, r, torch.bfloat16, sc), p3.apply_block(0, r, torch.bfloat16, sc))
+    assert torch.equal(p2.apply(r), p3.apply(r))
+
+
+def test_fused_fold_third_channel_solve(dev):
+    """npsd on the held-out 128^3 sphere scene with the shipped three-channel weights: the
+    folded BF16 plan converges (FP64-checked residual) in the iterations of the
+    three-channel BF16 plan, within one."""
+    from mlpcg_amd.sm_model_3d import SmallSMModelDn3D
+    from mlpcg_amd.solver import FusedPCG
+    sd = torch.load(os.path.join(ROOT, "mlpcg_amd", "weights", "sm_l5_c3.pt"), weights_only=True)
+    N = 128
+    m = SmallSMModelDn3D(5, 3, "trilinear").cuda()
+    m.load_state_dict(sd)
+    sc = S.scene_spheres(N, 7)
+    A = S.build_matrix(sc.flags)
+    b = S.make_rhs(sc.flags, 0)[0]
+    img = torch.as_tensor(S.image_channels(sc.flags, 3)).float().cuda()
+    assert torch.equal(img[2], img[0] + img[1])
+    s = FusedPCG(sc.flags, "flags", device=dev)
+    bv = s.vector(b)
+    its = {}
+    for fold in (False, True):
+        plan = m.make_preconditioner(img, torch.bfloat16, fold_third=fold)
+        x, info = s.solve(bv, plan, method="npsd", max_it=300, tol=1e-6)
+        assert info["status"] == 0, (fold, info)
+        xs = x.cpu().numpy()[sc.fluid_cells].astype(np.float64)
+        assert np.linalg.norm(b - A @ xs) / np.linalg.norm(b) < 1.05e-6
+        its[fold] = info["iters"]
+    assert abs(its[True] - its[False]) <= 1, its
+
+
 @pytest.mark.parametrize("C,shape", [(1, (64, 64, 64)), (2, (64, 32, 128)), (3, (64, 64, 64))])
 def test_block_forward_bf16_fused_exact(dev, C, shape):
     """The BF16 block forward (training and inference) on grids of 2^18 voxels or more runs

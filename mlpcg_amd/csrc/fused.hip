@@ -23,7 +23,9 @@
 // Tile: a block of 4 waves covers FI planes x 8 rows x 64 z-voxels (FI = 8, or 4 / 2
 // where 8-plane tiles give too few blocks); x and img of the tile plus a one-cell halo are
 // staged in LDS; a wave processes 64 z-voxels of one (plane, row) per step: 4 groups of
-// 16 voxels.
+// 16 voxels.  A pre block can also write the 2x2x2 means of its output (the next level's
+// input) from its epilogue.  The K writer (fused_kwrite_kernel) runs the first product
+// alone, operands swapped, and stores K for the input gradient of a BF16 block.
 #include <type_traits>
 
 #include "common.hpp"

@@ -6,8 +6,10 @@
 //               z = sum_{s,e} w[s,e] ybar[s,e] + b           sm_linear_3d_kernel.cu:185-186
 //   weight grad G_p(v) = sum_b a[b,v] c[b,v+d_p]              sm_block_3d_kernel.cu:121-229
 //               gW[p,s,e] = sum_v G_p(v) img[s,v+e], gb[p] = sum_v G_p(v)
-//               i.e. the [27 x V] . [V x (27C+1)] product of per-voxel outer products,
-//               tiled through LDS with a 3x4 register tile per thread.
+//               i.e. the [27 x V] . [V x (27C+1)] product of per-voxel outer products:
+//               on the matrix cores (FP32 operands for one channel; BF16 operands with
+//               exact hi/lo products for BF16 data, any C), else tiled through LDS with a
+//               3x4 register tile per thread.
 #include <mutex>
 
 #include "common.hpp"

@@ -157,6 +157,16 @@ int mlpcg_flags_code(const int8_t* flags, uint8_t* code, int n1, int n2, int n3,
 /* y = A x over the full grid, matrix-free from code; x must vanish off the unknowns. */
 int mlpcg_spmv_flags(const uint8_t* code, const float* x, float* y, int n1, int n2, int n3,
                      void* stream);
+/* No counterpart in the reference as a call: its training loss is torch arithmetic around
+ * a sparse product (loss_functions.py:18-23 residual_loss, train.py:61-72).  One right-hand
+ * side of the relative squared residual on the full grid, M = the cells with code != 0:
+ * r = b - M A M x (x of dtype MLPCG_F32 or MLPCG_BF16 need not vanish off M) and
+ * out2 = [r.r, b.b] in FP64 (workspace: mlpcg_dot_workspace_size() bytes); and the
+ * gradient g = ((float)*coef * A M r) * M stored as g_dtype (MLPCG_F32 or MLPCG_BF16). */
+int mlpcg_loss_resid(const void* x, int x_dtype, const float* b, const uint8_t* code, float* r, int n1,
+                     int n2, int n3, double* out2, void* workspace, void* stream);
+int mlpcg_loss_grad(const float* r, const uint8_t* code, const double* coef, void* g, int g_dtype, int n1,
+                    int n2, int n3, void* stream);
 /* y = A x, CSR with int32 indices; dtype MLPCG_F32 or MLPCG_F64 (val, x, y alike). */
 int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const void* x, void* y,
                    long n, int dtype, void* stream);

@@ -320,6 +320,23 @@ int mlpcg_spmv_flags(const uint8_t* code, const float* x, float* y, int n1, int 
   return launch_spmv_flat(code, x, y, n1, n2, n3, STREAM(stream));
 }
 
+int mlpcg_loss_resid(const void* x, int x_dtype, const float* b, const uint8_t* code, float* r, int n1, int n2,
+                     int n3, double* out2, void* workspace, void* stream) {
+  MLPCG_REQUIRE(x && b && code && r && out2 && workspace, "loss_resid: null pointer");
+  MLPCG_REQUIRE(aligned16(x) && aligned16(b) && aligned16(code) && aligned16(r) && aligned16(workspace),
+                "loss_resid: pointers must be 16-byte aligned");
+  MLPCG_REQUIRE(n1 >= 1 && n2 >= 1 && n3 >= 1, "loss_resid: bad grid (%d,%d,%d)", n1, n2, n3);
+  return launch_loss_resid(x_dtype, x, b, code, r, n1, n2, n3, (double*)workspace, out2, STREAM(stream));
+}
+
+int mlpcg_loss_grad(const float* r, const uint8_t* code, const double* coef, void* g, int g_dtype, int n1, int n2,
+                    int n3, void* stream) {
+  MLPCG_REQUIRE(r && code && coef && g, "loss_grad: null pointer");
+  MLPCG_REQUIRE(aligned16(r) && aligned16(code) && aligned16(g), "loss_grad: pointers must be 16-byte aligned");
+  MLPCG_REQUIRE(n1 >= 1 && n2 >= 1 && n3 >= 1, "loss_grad: bad grid (%d,%d,%d)", n1, n2, n3);
+  return launch_loss_grad(g_dtype, r, code, coef, g, n1, n2, n3, STREAM(stream));
+}
+
 int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const void* x, void* y, long n,
                    int dtype, void* stream) {
   MLPCG_REQUIRE(rowptr && col && val && x && y, "spmv_csr: null pointer");

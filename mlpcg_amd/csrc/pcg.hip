@@ -327,6 +327,101 @@ __global__ __launch_bounds__(kThreads) void spmv_flat_kernel(const uint8_t* __re
   store_vec<VZ>(y + r.v0, out);
 }
 
+// ---------------------------------------------------------------------------
+// Training loss ||b - M A M x||^2 / ||b||^2 of one right-hand side (M: the fluid cells,
+// code != 0) for a network output x on the full grid that need not vanish off the fluid:
+// the residual with the partial sums of r.r and b.b, and the gradient c M A M r.  The same
+// arithmetic as x.float() * fluid, the matrix-free A, b - A x and (c * A(r * fluid)) *
+// fluid in torch, in one pass each.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(bf16 v) { return (float)v; }
+
+// x * (code != 0) on the 7-point neighbourhood of a run (layout of load_star)
+template <int VZ, typename TX>
+__device__ __forceinline__ void load_masked_star(const TX* __restrict__ x, const uint8_t* __restrict__ code,
+                                                 const Run<VZ>& r, int n1, int n2, int n3,
+                                                 float (&c)[VZ + 2], float (&s)[4][VZ]) {
+  const long s1 = (long)n2 * n3;
+  auto at = [&](long v) { return code[v] ? to_f(x[v]) : 0.f; };
+  TX xv[VZ];
+  uint8_t cv[VZ];
+  load_vec<VZ>(x + r.v0, xv);
+  load_vec<VZ>(code + r.v0, cv);
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) c[t + 1] = cv[t] ? to_f(xv[t]) : 0.f;
+  c[0] = r.k0 > 0 ? at(r.v0 - 1) : 0.f;
+  c[VZ + 1] = r.k0 + VZ < n3 ? at(r.v0 + VZ) : 0.f;
+  const long offs[4] = {-s1, s1, -(long)n3, (long)n3};
+  const bool ok[4] = {r.i > 0, r.i < n1 - 1, r.j > 0, r.j < n2 - 1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (ok[q]) {
+      load_vec<VZ>(x + r.v0 + offs[q], xv);
+      load_vec<VZ>(code + r.v0 + offs[q], cv);
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) s[q][t] = cv[t] ? to_f(xv[t]) : 0.f;
+    } else {
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) s[q][t] = 0.f;
+    }
+  }
+}
+
+// r = b - A (M x); partials [r.r, b.b]
+template <int VZ, typename TX>
+__global__ __launch_bounds__(kThreads) void loss_resid_flat_kernel(const TX* __restrict__ x,
+                                                                   const float* __restrict__ b,
+                                                                   const uint8_t* __restrict__ code,
+                                                                   float* __restrict__ r, int n1, int n2,
+                                                                   int n3, double* partials) {
+  double red[2] = {0.0, 0.0};
+  const long nch = (long)n1 * n2 * (n3 / VZ);
+  for (long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; cid < nch;
+       cid += (long)gridDim.x * kThreads) {
+    Run<VZ> ru;
+    run_at<VZ>(cid, n1, n2, n3, ru);
+    float c[VZ + 2], s[4][VZ];
+    load_masked_star<VZ, TX>(x, code, ru, n1, n2, n3, c, s);
+    float bv[VZ], rv[VZ];
+    uint8_t cd[VZ];
+    load_vec<VZ>(b + ru.v0, bv);
+    load_vec<VZ>(code + ru.v0, cd);
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) {
+      const float ax = cd[t] ? lap_at<VZ, float>(c, s, t, (int)cd[t]) : 0.f;
+      rv[t] = bv[t] - ax;
+      red[0] += (double)rv[t] * (double)rv[t];
+      red[1] += (double)bv[t] * (double)bv[t];
+    }
+    store_vec<VZ>(r + ru.v0, rv);
+  }
+  block_partials<2>(red, partials);
+}
+
+// g = ((float)coef * A (M r)) * M, stored in the activation type
+template <int VZ, typename TG>
+__global__ __launch_bounds__(kThreads) void loss_grad_flat_kernel(const float* __restrict__ r,
+                                                                  const uint8_t* __restrict__ code,
+                                                                  const double* __restrict__ coef,
+                                                                  TG* __restrict__ g, int n1, int n2, int n3) {
+  const float cf = (float)coef[0];
+  Run<VZ> ru;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (!run_at<VZ>(cid, n1, n2, n3, ru)) return;
+  float c[VZ + 2], s[4][VZ];
+  load_masked_star<VZ, float>(r, code, ru, n1, n2, n3, c, s);
+  uint8_t cd[VZ];
+  load_vec<VZ>(code + ru.v0, cd);
+  TG out[VZ];
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) {
+    const float ax = cd[t] ? lap_at<VZ, float>(c, s, t, (int)cd[t]) : 0.f;
+    out[t] = (TG)((cf * ax) * (cd[t] ? 1.f : 0.f));
+  }
+  store_vec<VZ>(g + ru.v0, out);
+}
+
 // sum_i a[i] b[i] in FP64 (drop-in dot for torch-free callers).
 __global__ __launch_bounds__(kThreads) void dot_kernel(const float* __restrict__ x,
                                                        const float* __restrict__ y, long n,
@@ -513,7 +608,7 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_csr_kernel(CsrArgs a, const
 // Finalisation of the two-kernel reductions: one block sums the partials in block order
 // and updates the solver scalars.
 // ---------------------------------------------------------------------------
-enum FinOp : int { kFinResidInit = 0, kFinResid, kFinPW, kFinRZ, kFinDot, kFinProj, kFinNpsd, kFinRZ2 };
+enum FinOp : int { kFinResidInit = 0, kFinResid, kFinPW, kFinRZ, kFinDot, kFinProj, kFinNpsd, kFinRZ2, kFinDot2 };
 
 // aux/aux2: the output of kFinDot; alfa of N and O for kFinProj; alfa of the new
 // direction for kFinNpsd.
@@ -521,7 +616,7 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
                                                                const double* __restrict__ partials,
                                                                unsigned nb, SolveState* st,
                                                                double* aux, const double* aux2) {
-  if (op != kFinResidInit && op != kFinDot && st->done) return;  // uniform over the block
+  if (op != kFinResidInit && op != kFinDot && op != kFinDot2 && st->done) return;  // uniform over the block
   // thread 0 fetches the scalars it will read while the block sums the partials
   double rho = 0.0, rho_old = 0.0, norm_b = 0.0, tol = 0.0, atol = 0.0, cross = 0.0, an = 0.0, ao = 0.0;
   int iter = 0, max_it = 0, method = 0;
@@ -584,6 +679,10 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
     }
     case kFinDot:
       aux[0] = t[0];
+      break;
+    case kFinDot2:
+      aux[0] = t[0];
+      aux[1] = t[1];
       break;
     case kFinProj: {
       const double gN = t[0] / an;
@@ -742,6 +841,45 @@ int launch_spmv_flat(const uint8_t* code, const float* x, float* y, int n1, int 
   else if (vz == 2) spmv_flat_kernel<2><<<nb, kThreads, 0, st>>>(code, x, y, n1, n2, n3);
   else spmv_flat_kernel<1><<<nb, kThreads, 0, st>>>(code, x, y, n1, n2, n3);
   return check_launch("spmv_flat");
+}
+
+int launch_loss_resid(int tx, const void* x, const float* b, const uint8_t* code, float* r, int n1, int n2,
+                      int n3, double* partials, double* out2, hipStream_t st) {
+  const int vz = flat_vz(n3);
+  const long items = (long)n1 * n2 * (n3 / vz);
+  const unsigned nb = red_blocks(items);
+#define MLPCG_LR(TX)                                                                                   \
+  do {                                                                                                 \
+    if (vz == 8) loss_resid_flat_kernel<8, TX><<<nb, kThreads, 0, st>>>((const TX*)x, b, code, r, n1, n2, n3, partials); \
+    else if (vz == 4) loss_resid_flat_kernel<4, TX><<<nb, kThreads, 0, st>>>((const TX*)x, b, code, r, n1, n2, n3, partials); \
+    else if (vz == 2) loss_resid_flat_kernel<2, TX><<<nb, kThreads, 0, st>>>((const TX*)x, b, code, r, n1, n2, n3, partials); \
+    else loss_resid_flat_kernel<1, TX><<<nb, kThreads, 0, st>>>((const TX*)x, b, code, r, n1, n2, n3, partials); \
+  } while (0)
+  if (tx == kBF16) MLPCG_LR(bf16);
+  else if (tx == kF32) MLPCG_LR(float);
+  else MLPCG_REQUIRE(false, "loss_resid: x must be FP32 or BF16 (got %d)", tx);
+#undef MLPCG_LR
+  finalize(kFinDot2, 2, partials, nb, nullptr, out2, st);
+  return check_launch("loss_resid");
+}
+
+int launch_loss_grad(int tg, const float* r, const uint8_t* code, const double* coef, void* g, int n1, int n2,
+                     int n3, hipStream_t st) {
+  const int vz = flat_vz(n3);
+  const long items = (long)n1 * n2 * (n3 / vz);
+  const unsigned nb = (unsigned)((items + kThreads - 1) / kThreads);
+#define MLPCG_LG(TG)                                                                                  \
+  do {                                                                                                \
+    if (vz == 8) loss_grad_flat_kernel<8, TG><<<nb, kThreads, 0, st>>>(r, code, coef, (TG*)g, n1, n2, n3);       \
+    else if (vz == 4) loss_grad_flat_kernel<4, TG><<<nb, kThreads, 0, st>>>(r, code, coef, (TG*)g, n1, n2, n3);  \
+    else if (vz == 2) loss_grad_flat_kernel<2, TG><<<nb, kThreads, 0, st>>>(r, code, coef, (TG*)g, n1, n2, n3);  \
+    else loss_grad_flat_kernel<1, TG><<<nb, kThreads, 0, st>>>(r, code, coef, (TG*)g, n1, n2, n3);               \
+  } while (0)
+  if (tg == kBF16) MLPCG_LG(bf16);
+  else if (tg == kF32) MLPCG_LG(float);
+  else MLPCG_REQUIRE(false, "loss_grad: g must be FP32 or BF16 (got %d)", tg);
+#undef MLPCG_LG
+  return check_launch("loss_grad");
 }
 
 int launch_dot(const float* x, const float* y, long n, double* partials,

@@ -87,6 +87,13 @@ int launch_spmv_csr_f64(const int* rowptr, const int* col, const double* val, co
                         double* y, long n, hipStream_t st);
 int launch_spmv_flat(const uint8_t* code, const float* x, float* y, int n1, int n2, int n3,
                      hipStream_t st);
+// Training loss of one right-hand side on the full grid (M: cells with code != 0):
+// r = b - M A M x (x FP32 or BF16) and out2 = [r.r, b.b] (FP64, two-kernel reduction over
+// `partials`); g = ((float)*coef * A M r) * M in the activation type (FP32 or BF16).
+int launch_loss_resid(int tx, const void* x, const float* b, const uint8_t* code, float* r, int n1, int n2,
+                      int n3, double* partials, double* out2, hipStream_t st);
+int launch_loss_grad(int tg, const float* r, const uint8_t* code, const double* coef, void* g, int n1, int n2,
+                     int n3, hipStream_t st);
 int launch_dot(const float* x, const float* y, long n, double* partials,
                double* out, hipStream_t st);
 

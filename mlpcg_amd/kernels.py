@@ -218,6 +218,33 @@ def spmv_flags(code, x):
     return y
 
 
+def loss_resid(code, x, b):
+    """Training loss of one right-hand side on the full grid: r = b - M A M x (x float32 or
+    bfloat16, any values off the fluid cells M = code != 0) and the (2,) float64 device
+    tensor [r.r, b.b]."""
+    require_cuda(code, x, b)
+    n1, n2, n3 = _grid(code)
+    x = x.contiguous()
+    b = b.float().contiguous()
+    r = torch.empty(n1 * n2 * n3, dtype=torch.float32, device=x.device)
+    out = torch.empty(2, dtype=torch.float64, device=x.device)
+    check(lib.mlpcg_loss_resid(ptr(x), code_of(x.dtype), ptr(b), ptr(code), ptr(r), n1, n2, n3, ptr(out),
+                               ptr(_dot_workspace(x.device)), stream_ptr(x.device)), "loss_resid")
+    return r, out
+
+
+def loss_grad(code, r, coef, dtype):
+    """g = ((float)coef * A M r) * M on the full grid, as `dtype` (float32 or bfloat16);
+    coef: 0-dim float64 device tensor."""
+    require_cuda(code, r, coef)
+    n1, n2, n3 = _grid(code)
+    coef = coef.reshape(1).contiguous()
+    g = torch.empty(n1 * n2 * n3, dtype=dtype, device=r.device)
+    check(lib.mlpcg_loss_grad(ptr(r.contiguous()), ptr(code), ptr(coef), ptr(g), code_of(dtype), n1, n2, n3,
+                              stream_ptr(r.device)), "loss_grad")
+    return g
+
+
 def spmv_csr(rowptr, col, val, x):
     require_cuda(rowptr, col, val, x)
     y = torch.empty_like(x)
@@ -229,14 +256,19 @@ def spmv_csr(rowptr, col, val, x):
 _dot_ws = {}
 
 
-def dot(x, y):
-    """FP64-accumulated dot of two float32 vectors -> 0-dim float64 tensor (device)."""
-    require_cuda(x, y)
-    dev = x.device
+def _dot_workspace(dev):
     ws = _dot_ws.get(dev)
     if ws is None:
         ws = torch.zeros(lib.mlpcg_dot_workspace_size(), dtype=torch.uint8, device=dev)
         _dot_ws[dev] = ws
+    return ws
+
+
+def dot(x, y):
+    """FP64-accumulated dot of two float32 vectors -> 0-dim float64 tensor (device)."""
+    require_cuda(x, y)
+    dev = x.device
+    ws = _dot_workspace(dev)
     out = torch.empty((), dtype=torch.float64, device=dev)
     check(lib.mlpcg_dot(ptr(x), ptr(y), x.numel(), ptr(out), ptr(ws), stream_ptr(dev)), "dot")
     return out

@@ -115,31 +115,45 @@ def relative_residual_sq(sb: SceneBatch, x, b):
 
 class RelResidualSqFunction(torch.autograd.Function):
     """mean_b ||b_b - M A M x_b||^2 / ||b_b||^2 (M: restriction to the fluid cells) with the
-    reductions done by the library's two-kernel FP64 dot instead of torch reductions, so a
+    reductions done by the library's two-kernel FP64 dots instead of torch reductions, so a
     training step built on it can be captured in a HIP graph.  A is symmetric, hence
-    d/dx_b = -2 / (B ||b_b||^2) M A M r_b."""
+    d/dx_b = -2 / (B ||b_b||^2) M A M r_b.  For FP32 and BF16 activations each term is two
+    library kernels (residual with its sums; gradient in the activation type), the same
+    arithmetic as x.float() * fluid, the matrix-free A, b - A x, (c * A(r * fluid)) * fluid."""
 
     @staticmethod
     def forward(ctx, x, b, code, fluid):
         B, V = b.shape
-        xf = (x.reshape(B, V).float() * fluid).contiguous()
         b = b.contiguous()
-        r = torch.stack([bi - K.spmv_flags(code, xi) for bi, xi in zip(b, xf)])
-        nr = torch.stack([K.dot(ri, ri) for ri in r])
-        nb = torch.stack([K.dot(bi, bi) for bi in b])
-        loss = (nr / nb).sum()  # B entries: one workgroup, no partials to clear
-        ctx.save_for_backward(r, nb, code, fluid)
         ctx.xshape, ctx.xdtype = x.shape, x.dtype
+        ctx.fused = x.dtype in (torch.float32, torch.bfloat16)
+        if ctx.fused:
+            xs = x.reshape(B, V)
+            terms = [K.loss_resid(code, xs[i], b[i]) for i in range(B)]
+            rs = [t[0] for t in terms]
+            nr = torch.stack([t[1][0] for t in terms])
+            nb = torch.stack([t[1][1] for t in terms])
+        else:
+            xf = (x.reshape(B, V).float() * fluid).contiguous()
+            rs = [bi - K.spmv_flags(code, xi) for bi, xi in zip(b, xf)]
+            nr = torch.stack([K.dot(ri, ri) for ri in rs])
+            nb = torch.stack([K.dot(bi, bi) for bi in b])
+        loss = (nr / nb).sum()  # B entries: one workgroup, no partials to clear
+        ctx.save_for_backward(nb, code, fluid, *rs)
         return (loss / B).float()
 
     @staticmethod
     def backward(ctx, g):
-        r, nb, code, fluid = ctx.saved_tensors
-        B = r.shape[0]
+        nb, code, fluid, *rs = ctx.saved_tensors
+        B = len(rs)
         coef = (-2.0 / B) * g.double() / nb  # (B,)
-        gx = torch.stack([(c.float() * K.spmv_flags(code, (ri * fluid).contiguous())) * fluid
-                          for c, ri in zip(coef, r)])
-        return gx.reshape(ctx.xshape).to(ctx.xdtype), None, None, None
+        if ctx.fused:
+            gs = [K.loss_grad(code, ri, c, ctx.xdtype) for c, ri in zip(coef, rs)]
+        else:
+            gs = [((c.float() * K.spmv_flags(code, (ri * fluid).contiguous())) * fluid).to(ctx.xdtype)
+                  for c, ri in zip(coef, rs)]
+        gx = gs[0] if B == 1 else torch.stack(gs)
+        return gx.reshape(ctx.xshape), None, None, None
 
 
 def relative_residual_sq_dot(sb: SceneBatch, x, b):

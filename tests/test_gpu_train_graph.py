@@ -22,20 +22,24 @@ def _setup(dev, n=32, levels=3, scenes=2, B=2):
     return T, model, sbs, gen, B
 
 
-def test_dot_loss_matches_torch_loss_and_gradient(dev):
+@pytest.mark.parametrize("act,gtol", [(torch.float32, 1e-5), (torch.bfloat16, 8e-3)])
+def test_dot_loss_matches_torch_loss_and_gradient(dev, act, gtol):
+    """The library loss (fused residual and gradient kernels for FP32 / BF16 outputs) against
+    the torch composition on the same network output: the loss to FP64-sum order, the
+    gradient to FP32 order (FP32) or to the BF16 rounding of the gradient (BF16)."""
     T, model, sbs, gen, B = _setup(dev)
     sb = sbs[0]
     b = T.white_rhs(sb, B, gen)
-    x0 = model(sb.image, b.reshape(B, 1, *sb.shape)).detach()
+    x0 = model(sb.image, b.reshape(B, 1, *sb.shape)).detach().to(act)
     vals, grads = [], []
     for fn in (T.relative_residual_sq, T.relative_residual_sq_dot):
         x = x0.clone().requires_grad_()
         loss = fn(sb, x, b)
         loss.backward()
         vals.append(float(loss.detach()))
-        grads.append(x.grad.detach().clone())
+        grads.append(x.grad.detach().clone().float())
     assert abs(vals[0] - vals[1]) <= 1e-5 * max(1.0, abs(vals[0]))
-    assert float((grads[0] - grads[1]).norm() / grads[0].norm()) < 1e-5
+    assert float((grads[0] - grads[1]).norm() / grads[0].norm()) < gtol
 
 
 @pytest.mark.parametrize("act", [torch.float32, torch.bfloat16])

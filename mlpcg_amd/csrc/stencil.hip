@@ -577,6 +577,69 @@ __global__ __launch_bounds__(kThreads) void upsample2_adjoint_kernel(const TI* _
   gout[b * Vc + id] = (TO)(acc * (A)scale);
 }
 
+// The same gather for runs of VZ coarse cells along z (coarse n3 % VZ == 0): per fine row
+// of the 4 x 4 window one vector load of the 2 VZ fine cells under the run plus the two
+// cells either side, instead of 4 scalar loads per row and coarse cell; the weights and the
+// summation order (fine rows ascending, fine z ascending within a row) are those above.
+template <typename TI, typename TO, int VZ>
+__global__ __launch_bounds__(kThreads) void upsample2_adjoint_run_kernel(const TI* __restrict__ gin,
+                                                                         TO* __restrict__ gout, int B,
+                                                                         int n1, int n2, int n3,
+                                                                         int nearest, double scale) {
+  typedef typename AccOf<TO>::type A;
+  ChunkPos c;
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (!chunk_pos<VZ>(cid, n1, n2, n3, c)) return;
+  const int b = blockIdx.y;
+  const long Vc = (long)n1 * n2 * n3;
+  const int f1 = 2 * n1, f2 = 2 * n2, f3 = 2 * n3;
+  const TI* __restrict__ g = gin + b * Vc * 8;
+  // z weights of the run: coarse cell k0 + t reads fine z 2(k0 + t) - 1 + d, d = 0..3
+  A wz[VZ][4];
+#pragma unroll
+  for (int t = 0; t < VZ; ++t)
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const int fk = 2 * (c.k0 + t) - 1 + d;
+      wz[t][d] = (fk >= 0 && fk < f3) ? (A)up_weight(fk, c.k0 + t, n3, nearest) : A(0);
+    }
+  const int z0 = 2 * c.k0;  // first fine cell under the run
+  A acc[VZ];
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) acc[t] = A(0);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int fi = 2 * c.i - 1 + a;
+    if (fi < 0 || fi >= f1) continue;
+    const A wi = (A)up_weight(fi, c.i, n1, nearest);
+    if (wi == A(0)) continue;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int fj = 2 * c.j - 1 + e;
+      if (fj < 0 || fj >= f2) continue;
+      const A wj = (A)up_weight(fj, c.j, n2, nearest);
+      if (wj == A(0)) continue;
+      const TI* row = g + ((long)fi * f2 + fj) * f3;
+      A f[2 * VZ + 2];  // fine z0 - 1 .. z0 + 2 VZ
+      load_vec<2 * VZ>(row + z0, f + 1);
+      f[0] = z0 > 0 ? (A)row[z0 - 1] : A(0);
+      f[2 * VZ + 1] = z0 + 2 * VZ < f3 ? (A)row[z0 + 2 * VZ] : A(0);
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) {
+        A sum = A(0);
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+          if (wz[t][d] != A(0)) sum += wz[t][d] * f[2 * t + d];
+        acc[t] += wi * wj * sum;
+      }
+    }
+  }
+  A out[VZ];
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) out[t] = acc[t] * (A)scale;
+  store_vec<VZ>(gout + b * Vc + c.v0, out);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -833,9 +896,29 @@ int launch_upsample2(int dtype, const UpArgs& args, hipStream_t st) {
   return check_launch("upsample2");
 }
 
+template <typename T>
+static bool launch_up_adj_runs(const void* gin, void* gout, int B, int n1, int n2, int n3, int nearest,
+                               double scale, hipStream_t st) {
+  const int vz = pick_vz_small(n3, 4, (long)n1 * n2 * n3 * B);
+  if (vz < 2) return false;
+  const long items = (long)n1 * n2 * (n3 / vz);
+  if (vz == 4)
+    upsample2_adjoint_run_kernel<T, T, 4><<<grid1(items, B), kThreads, 0, st>>>((const T*)gin, (T*)gout, B, n1, n2,
+                                                                              n3, nearest, scale);
+  else
+    upsample2_adjoint_run_kernel<T, T, 2><<<grid1(items, B), kThreads, 0, st>>>((const T*)gin, (T*)gout, B, n1, n2,
+                                                                              n3, nearest, scale);
+  return true;
+}
+
 int launch_upsample2_adjoint(int dtype, const void* gin, void* gout, int B, int n1, int n2, int n3,
                              int nearest, double scale, hipStream_t st) {
   const long items = (long)n1 * n2 * n3;
+  // runs of coarse cells where the coarse rows allow them (16-bit and FP32 vectors)
+  if ((dtype == kBF16 && launch_up_adj_runs<bf16>(gin, gout, B, n1, n2, n3, nearest, scale, st)) ||
+      (dtype == kF16 && launch_up_adj_runs<f16>(gin, gout, B, n1, n2, n3, nearest, scale, st)) ||
+      (dtype == kF32 && launch_up_adj_runs<float>(gin, gout, B, n1, n2, n3, nearest, scale, st)))
+    return check_launch("upsample2_adjoint");
   switch (dtype) {
     case kF64:
       upsample2_adjoint_kernel<double, double><<<grid1(items, B), kThreads, 0, st>>>(

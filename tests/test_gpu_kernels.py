@@ -256,6 +256,27 @@ def test_upsample2_adjoint_f64(dev, mode, shape):
     assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(lhs))
 
 
+@pytest.mark.parametrize("mode", ["trilinear", "nearest"])
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-6), (torch.bfloat16, 2 ** -8),
+                                       (torch.float16, 2 ** -11)])
+@pytest.mark.parametrize("shape", [(128, 128, 64), (64, 64, 130)])
+def test_upsample2_adjoint_runs(dev, mode, dtype, tol, shape):
+    """Coarse grids of >= 2^19 cells take the z-run gather (4 or 2 coarse cells per thread);
+    per element it equals the transpose of torch's upsampling in FP64 up to the output's
+    rounding."""
+    from mlpcg_amd import kernels as K
+    yf = _rand((1, 1, *(2 * s for s in shape)), 36).to(dtype)
+    xr = torch.zeros((1, 1, *shape), dtype=torch.float64, requires_grad=True)
+    up = F.interpolate(xr, scale_factor=2, mode=mode) if mode != "nearest" else F.interpolate(xr, scale_factor=2)
+    up.backward(yf.double())
+    ref = xr.grad * 0.5
+    out = K.upsample2_adjoint(yf.to(dev), mode, 0.5).double().cpu()
+    assert out.shape == ref.shape
+    err = (out - ref).abs()
+    # output rounding per element, FP32 accumulation over at most 64 products
+    assert float((err - tol * ref.abs()).max()) <= 1e-5 * float(ref.abs().max())
+
+
 @pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-6), (torch.bfloat16, 2 ** -7)])
 def test_sm_apply_epilogue(dev, dtype, tol):
     """y = c0 * prev + c1 * (scale * stencil), masked before and after."""

@@ -337,6 +337,21 @@ int mlpcg_loss_grad(const float* r, const uint8_t* code, const double* coef, voi
   return launch_loss_grad(g_dtype, r, code, coef, g, n1, n2, n3, STREAM(stream));
 }
 
+int mlpcg_comb(const void* x0, const void* x1, const void* c0, const void* c1, int c_dtype, void* y, long n,
+               int dtype, void* stream) {
+  MLPCG_REQUIRE(n >= 0, "comb: bad length %ld", n);
+  MLPCG_REQUIRE(n == 0 || (x0 && x1 && c0 && c1 && y), "comb: null pointer");
+  return launch_comb(dtype, x0, x1, c0, c1, c_dtype, y, n, STREAM(stream));
+}
+
+int mlpcg_comb_backward(const void* g, const void* x0, const void* x1, const void* c0, const void* c1, int c_dtype,
+                        void* g0, void* g1, long n, int dtype, double* out2, void* workspace, void* stream) {
+  MLPCG_REQUIRE(n >= 0, "comb_backward: bad length %ld", n);
+  MLPCG_REQUIRE(c0 && c1 && out2 && workspace && (n == 0 || (g && x0 && x1)), "comb_backward: null pointer");
+  MLPCG_REQUIRE(aligned16(workspace), "comb_backward: workspace must be 16-byte aligned");
+  return launch_comb_grad(dtype, g, x0, x1, c0, c1, c_dtype, g0, g1, n, (double*)workspace, out2, STREAM(stream));
+}
+
 int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const void* x, void* y, long n,
                    int dtype, void* stream) {
   MLPCG_REQUIRE(rowptr && col && val && x && y, "spmv_csr: null pointer");

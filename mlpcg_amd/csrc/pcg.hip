@@ -11,6 +11,8 @@
 // them in block order, see common.hpp); the finalize updates the solver scalars in
 // SolveState on the device, so an iteration needs no host round trip.  Every kernel
 // returns at once when state->done is set.
+#include <initializer_list>
+
 #include "common.hpp"
 #include "pcg.hpp"
 
@@ -430,6 +432,67 @@ __global__ __launch_bounds__(kThreads) void dot_kernel(const float* __restrict__
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads)
     red[0] += (double)x[i] * (double)y[i];
   block_partials<1>(red, partials);
+}
+
+// Level combination of the SM models' upward pass, y = c0 x0 + c1 x1, with torch's
+// rounding of those expressions: each coefficient is first rounded to the element type,
+// each product and the sum are rounded to it (FP32 products are single roundings, BF16
+// products of BF16 operands are exact in FP32 and rounded once to BF16).
+__device__ __forceinline__ float coef_as(const void* c, int ct) {
+  return ct == kBF16 ? (float)*(const bf16*)c : *(const float*)c;
+}
+template <typename T>
+__device__ __forceinline__ float rnd(float v) {
+  return (float)(T)v;
+}
+template <int VZ, typename T>
+__global__ __launch_bounds__(kThreads) void comb_kernel(const T* __restrict__ x0, const T* __restrict__ x1,
+                                                        const void* c0p, const void* c1p, int ct,
+                                                        T* __restrict__ y, long nch) {
+#pragma clang fp contract(off)  // torch rounds each product before the sum
+  const long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (cid >= nch) return;
+  const float c0 = rnd<T>(coef_as(c0p, ct)), c1 = rnd<T>(coef_as(c1p, ct));
+  float a[VZ], b[VZ], o[VZ];
+  load_vec<VZ>(x0 + cid * VZ, a);
+  load_vec<VZ>(x1 + cid * VZ, b);
+#pragma unroll
+  for (int t = 0; t < VZ; ++t) o[t] = rnd<T>(c0 * a[t]) + rnd<T>(c1 * b[t]);
+  store_vec<VZ>(y + cid * VZ, o);
+}
+
+// Its backward: g0 = c0 g, g1 = c1 g (rounded like the forward's products; either may be
+// null) and the coefficient gradients [g.x0, g.x1] accumulated in FP64.
+template <int VZ, typename T>
+__global__ __launch_bounds__(kThreads) void comb_grad_kernel(const T* __restrict__ g, const T* __restrict__ x0,
+                                                             const T* __restrict__ x1, const void* c0p,
+                                                             const void* c1p, int ct, T* __restrict__ g0,
+                                                             T* __restrict__ g1, long nch, double* partials) {
+  const float c0 = rnd<T>(coef_as(c0p, ct)), c1 = rnd<T>(coef_as(c1p, ct));
+  double red[2] = {0.0, 0.0};
+  for (long cid = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x; cid < nch;
+       cid += (long)gridDim.x * kThreads) {
+    float gv[VZ], a[VZ], b[VZ], o[VZ];
+    load_vec<VZ>(g + cid * VZ, gv);
+    load_vec<VZ>(x0 + cid * VZ, a);
+    load_vec<VZ>(x1 + cid * VZ, b);
+#pragma unroll
+    for (int t = 0; t < VZ; ++t) {
+      red[0] += (double)gv[t] * (double)a[t];
+      red[1] += (double)gv[t] * (double)b[t];
+    }
+    if (g0) {
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) o[t] = c0 * gv[t];
+      store_vec<VZ>(g0 + cid * VZ, o);
+    }
+    if (g1) {
+#pragma unroll
+      for (int t = 0; t < VZ; ++t) o[t] = c1 * gv[t];
+      store_vec<VZ>(g1 + cid * VZ, o);
+    }
+  }
+  block_partials<2>(red, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -880,6 +943,54 @@ int launch_loss_grad(int tg, const float* r, const uint8_t* code, const double* 
   else MLPCG_REQUIRE(false, "loss_grad: g must be FP32 or BF16 (got %d)", tg);
 #undef MLPCG_LG
   return check_launch("loss_grad");
+}
+
+// VZ of the combination kernels: 16-byte vectors where n and every pointer allow them
+static int comb_vz(int dt, long n, std::initializer_list<const void*> ptrs) {
+  int vz = dt == kBF16 ? 8 : 4;
+  for (const void* p : ptrs)
+    if (p && ((uintptr_t)p % 16) != 0) vz = 1;
+  return n % vz == 0 ? vz : 1;
+}
+
+int launch_comb(int dt, const void* x0, const void* x1, const void* c0, const void* c1, int ct, void* y, long n,
+                hipStream_t st) {
+  MLPCG_REQUIRE(ct == kF32 || ct == kBF16, "comb: coefficients must be FP32 or BF16 (got %d)", ct);
+  if (n == 0) return 0;
+  const int vz = comb_vz(dt, n, {x0, x1, y});
+  const long nch = n / vz;
+  const dim3 grid((unsigned)((nch + kThreads - 1) / kThreads));
+#define MLPCG_CB(T, V) comb_kernel<V, T><<<grid, kThreads, 0, st>>>((const T*)x0, (const T*)x1, c0, c1, ct, (T*)y, nch)
+  if (dt == kBF16) {
+    if (vz == 8) MLPCG_CB(bf16, 8); else MLPCG_CB(bf16, 1);
+  } else if (dt == kF32) {
+    if (vz == 4) MLPCG_CB(float, 4); else MLPCG_CB(float, 1);
+  } else {
+    MLPCG_REQUIRE(false, "comb: elements must be FP32 or BF16 (got %d)", dt);
+  }
+#undef MLPCG_CB
+  return check_launch("comb");
+}
+
+int launch_comb_grad(int dt, const void* g, const void* x0, const void* x1, const void* c0, const void* c1, int ct,
+                     void* g0, void* g1, long n, double* partials, double* out2, hipStream_t st) {
+  MLPCG_REQUIRE(ct == kF32 || ct == kBF16, "comb_grad: coefficients must be FP32 or BF16 (got %d)", ct);
+  const int vz = comb_vz(dt, n, {g, x0, x1, g0, g1});
+  const long nch = n / vz;
+  const unsigned nb = red_blocks(nch);
+#define MLPCG_CG(T, V)                                                                                    \
+  comb_grad_kernel<V, T><<<nb, kThreads, 0, st>>>((const T*)g, (const T*)x0, (const T*)x1, c0, c1, ct, (T*)g0, \
+                                                  (T*)g1, nch, partials)
+  if (dt == kBF16) {
+    if (vz == 8) MLPCG_CG(bf16, 8); else MLPCG_CG(bf16, 1);
+  } else if (dt == kF32) {
+    if (vz == 4) MLPCG_CG(float, 4); else MLPCG_CG(float, 1);
+  } else {
+    MLPCG_REQUIRE(false, "comb_grad: elements must be FP32 or BF16 (got %d)", dt);
+  }
+#undef MLPCG_CG
+  finalize(kFinDot2, 2, partials, nb, nullptr, out2, st);
+  return check_launch("comb_grad");
 }
 
 int launch_dot(const float* x, const float* y, long n, double* partials,

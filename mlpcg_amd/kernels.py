@@ -245,6 +245,45 @@ def loss_grad(code, r, coef, dtype):
     return g
 
 
+def comb(c0, x0, c1, x1):
+    """c0 * x0 + c1 * x1 for float32 / bfloat16 x0, x1 of one shape and one-element
+    coefficient tensors (float32 or bfloat16), with torch's roundings of that expression."""
+    require_cuda(c0, x0, c1, x1)
+    x0, x1 = x0.contiguous(), x1.contiguous()
+    if x0.shape != x1.shape or x0.dtype != x1.dtype:
+        raise ValueError("comb: x0 and x1 must have one shape and dtype")
+    c0, c1 = _coefs(c0, c1)
+    y = torch.empty_like(x0)
+    check(lib.mlpcg_comb(ptr(x0), ptr(x1), ptr(c0), ptr(c1), code_of(c0.dtype), ptr(y), x0.numel(),
+                         code_of(x0.dtype), stream_ptr(x0.device)), "comb")
+    return y
+
+
+def comb_backward(g, c0, x0, c1, x1, need_g0=True, need_g1=True):
+    """Backward of comb: (c0 * g or None, c1 * g or None, (2,) float64 [g.x0, g.x1])."""
+    require_cuda(g, c0, x0, c1, x1)
+    g, x0, x1 = g.contiguous(), x0.contiguous(), x1.contiguous()
+    if not (g.shape == x0.shape == x1.shape and g.dtype == x0.dtype == x1.dtype):
+        raise ValueError("comb_backward: g, x0 and x1 must have one shape and dtype")
+    c0, c1 = _coefs(c0, c1)
+    g0 = torch.empty_like(g) if need_g0 else None
+    g1 = torch.empty_like(g) if need_g1 else None
+    out = torch.empty(2, dtype=torch.float64, device=g.device)
+    check(lib.mlpcg_comb_backward(ptr(g), ptr(x0), ptr(x1), ptr(c0), ptr(c1), code_of(c0.dtype),
+                                  ptr(g0) if need_g0 else None, ptr(g1) if need_g1 else None, g.numel(),
+                                  code_of(g.dtype), ptr(out), ptr(_dot_workspace(g.device)),
+                                  stream_ptr(g.device)), "comb_backward")
+    return g0, g1, out
+
+
+def _coefs(c0, c1):
+    if c0.numel() != 1 or c1.numel() != 1:
+        raise ValueError("comb: coefficients must have one element")
+    if c0.dtype != c1.dtype:
+        c1 = c1.to(c0.dtype)
+    return c0.contiguous(), c1.contiguous()
+
+
 def spmv_csr(rowptr, col, val, x):
     require_cuda(rowptr, col, val, x)
     y = torch.empty_like(x)

@@ -230,20 +230,28 @@ def upsample(x, mode="trilinear", scale=1.0):
 
 class _CombFunction(torch.autograd.Function):
     """c0 * x0 + c1 * x1 with coefficients of shape (1,) (sm_model_3d.py:245,258,282,286).
-    The coefficient gradients are dot products over the whole level; for FP32/BF16 they go
-    through the library's deterministic FP64 dot, so a training step that contains them
-    can be captured in a HIP graph (a torch reduction over many workgroups records memset
-    nodes, which give wrong results on replays after the first)."""
+    FP32/BF16 levels run it as one library kernel each way (forward: one pass with torch's
+    roundings of the expression; backward: c0 g, c1 g and the coefficient gradients g.x0,
+    g.x1 in FP64 from one pass), instead of three and six torch kernels.  The coefficient
+    gradients are deterministic two-kernel reductions, so a training step that contains
+    them can be captured in a HIP graph (a torch reduction over many workgroups records
+    memset nodes, which give wrong results on replays after the first)."""
 
     @staticmethod
     def forward(ctx, c0, x0, c1, x1):
         ctx.save_for_backward(c0, x0, c1, x1)
+        if _comb_native(c0, x0, c1, x1):
+            return K.comb(c0, x0, c1, x1)
         return c0.to(x0.dtype) * x0 + c1.to(x1.dtype) * x1
 
     @staticmethod
     def backward(ctx, g):
         c0, x0, c1, x1 = ctx.saved_tensors
         need = ctx.needs_input_grad
+        if _comb_native(c0, x0, c1, x1) and g.dtype == x0.dtype:
+            g0, g1, dots = K.comb_backward(g, c0, x0, c1, x1, need[1], need[3])
+            return (dots[0:1].reshape(c0.shape).to(c0.dtype) if need[0] else None, g0,
+                    dots[1:2].reshape(c1.shape).to(c1.dtype) if need[2] else None, g1)
 
         def coef_grad(x, c):
             if x.dtype == torch.float64:
@@ -255,6 +263,15 @@ class _CombFunction(torch.autograd.Function):
                 c0.to(g.dtype) * g if need[1] else None,
                 coef_grad(x1, c1) if need[2] else None,
                 c1.to(g.dtype) * g if need[3] else None)
+
+
+def _comb_native(c0, x0, c1, x1):
+    """The library's combination kernels cover FP32 / BF16 levels with FP32 / BF16
+    coefficients (FP64 stays on torch, where the oracle comparisons run)."""
+    return (x0.is_cuda and x0.dtype in (torch.float32, torch.bfloat16) and x1.dtype == x0.dtype
+            and x0.shape == x1.shape
+            and c0.dtype in (torch.float32, torch.bfloat16) and c1.dtype in (torch.float32, torch.bfloat16)
+            and c0.numel() == 1 and c1.numel() == 1)
 
 
 def _comb(c0, x0, c1, x1):

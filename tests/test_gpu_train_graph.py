@@ -22,6 +22,32 @@ def _setup(dev, n=32, levels=3, scenes=2, B=2):
     return T, model, sbs, gen, B
 
 
+@pytest.mark.parametrize("dt", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("cdt", [torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("n,off", [(1 << 20, 0), (4099, 0), (4104, 1), (0, 0)])
+def test_comb_kernels_match_torch(dev, dt, cdt, n, off):
+    """The level combination kernels against torch's expression: forward and input
+    gradients bit-exact (same roundings), coefficient gradients against FP64 sums of the
+    exact products; vector (n % 8 == 0, aligned), ragged and misaligned (offset views)
+    lengths and the empty level."""
+    from mlpcg_amd import kernels as K
+    g = torch.Generator(device=dev).manual_seed(n + off)
+    mk = lambda: torch.randn(n + off, device=dev, generator=g).to(dt)[off:]
+    x0, x1, gr = mk(), mk(), mk()
+    c0 = torch.tensor([0.7318], device=dev).to(cdt)
+    c1 = torch.tensor([-1.2371], device=dev).to(cdt)
+    y = K.comb(c0, x0, c1, x1)
+    assert torch.equal(y, c0.to(dt) * x0 + c1.to(dt) * x1)
+    g0, g1, dots = K.comb_backward(gr, c0, x0, c1, x1)
+    assert torch.equal(g0, c0.to(dt) * gr) and torch.equal(g1, c1.to(dt) * gr)
+    ref = torch.stack([(gr.double() * x0.double()).sum(), (gr.double() * x1.double()).sum()])
+    scale = torch.stack([(gr.double() * x0.double()).abs().sum(), (gr.double() * x1.double()).abs().sum()])
+    assert bool(((dots - ref).abs() <= 1e-13 * scale + 1e-300).all())
+    # either input gradient may be skipped
+    h0, h1, d2 = K.comb_backward(gr, c0, x0, c1, x1, need_g0=False)
+    assert h0 is None and torch.equal(h1, g1) and torch.equal(d2, dots)
+
+
 @pytest.mark.parametrize("act,gtol", [(torch.float32, 1e-5), (torch.bfloat16, 8e-3)])
 def test_dot_loss_matches_torch_loss_and_gradient(dev, act, gtol):
     """The library loss (fused residual and gradient kernels for FP32 / BF16 outputs) against

@@ -219,9 +219,11 @@ class GraphedTrainStep:
                 _step_body(model, batches, self.rhs, act_dtype, loss_fn)
                 optimizer.step()
         torch.cuda.current_stream(dev).wait_stream(side)
+        # gradients unset at the capture: each parameter's first gradient of a replay is
+        # written in place of the old one (no fill, no accumulate into zeros)
+        optimizer.zero_grad(set_to_none=True)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            optimizer.zero_grad(set_to_none=False)
             self.loss = _step_body(model, batches, self.rhs, act_dtype, loss_fn)
             optimizer.step()
         with torch.no_grad():  # capture ran nothing: undo the warm-up steps

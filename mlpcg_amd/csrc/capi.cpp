@@ -132,7 +132,8 @@ static int block_bwd(bool trans, const void* g, const void* image, const void* x
   // here, not saved from the forward pass): for the block y = sum_p K_p(v) x(v + d_p) it
   // is the transposed gather sum_p K_p(v - d_p) g(v - d_p); for the transpose block it is
   // the block apply itself.  BF16 grids that the fused apply takes get K from its first
-  // product (matrix cores, an FP16 image copy), stored instead of applied.
+  // product (matrix cores, an FP16 image copy): the block sums g K along z-rows into nine
+  // column sums and gathers those; the transpose block stores K and applies it.
   if (dtype == kBF16 && V >= kFusedForwardMinVoxels && fused_eligible(C, B, n1, n2, n3)) {
     char* ext = (char*)part + align256(weight_grad_workspace(dtype, C, B, n1, n2, n3));
     void* img16 = ext;
@@ -146,6 +147,14 @@ static int block_bwd(bool trans, const void* g, const void* image, const void* x
     f.n1 = n1;
     f.n2 = n2;
     f.n3 = n3;
+    if (!trans) {
+      // the block's input gradient through the nine column sums of g K (in the K area)
+      MLPCG_REQUIRE(fused_sgrad_bytes(n1, n2, n3) <= kbytes, "%s: workspace layout", who);
+      f.x = g;
+      if (launch_fused_sgrad(C, f, workspace, gx, STREAM(stream))) return -1;
+      const void* av = g;
+      return launch_weight_grad(dtype, av, x, image, gw, gb, part, C, B, n1, n2, n3, STREAM(stream));
+    }
     if (launch_fused_kwrite(C, f, workspace, STREAM(stream))) return -1;
   } else if (launch_stencil_weights(dtype, image, weight, bias, workspace, C, n1, n2, n3, false,
                                     STREAM(stream))) {

@@ -748,6 +748,294 @@ __global__ __launch_bounds__(kThreads) void fused_kwrite_kernel(FusedArgs a, bf1
   }
 }
 
+// Input gradient of a BF16 block through column sums (the S path).  With P_p(v) = g(v) K_p(v)
+// the input gradient is dx(u) = sum_p P_p(u - d_p).  Grouping the 27 taps by their (d1, d2)
+// column c = 3 (d1 + 1) + (d2 + 1) and c' = 8 - c,
+//   S_c(v) = P_{3c'+2}(v - e3) + P_{3c'+1}(v) + P_{3c'}(v + e3)     (a sum along the z-row of v)
+//   dx(u)  = sum_c S_c(u + (c / 3 - 1, c % 3 - 1, 0)).
+// fused_swrite_kernel computes K on the matrix cores as the K writer does, multiplies by g,
+// forms the nine row sums in LDS and stores S (BF16, 9 x V) -- 18 bytes per voxel instead of
+// the 54 of K; the gather then reads 18 bytes per voxel instead of 56, and needs no z
+// neighbours.  The two z-end cells of a tile row miss one term from the neighbouring tile:
+// their partial sums and the terms the neighbours miss go to a side array (FP32, 36 values
+// per tile row) that the gather combines.  S is rounded once to BF16 (the stored-K path
+// rounds K instead); the sums are FP32.
+constexpr int kPsPitch = 68;  // floats per tap row of a wave's P buffer (64 cells + pad)
+
+template <int C, int FI>
+__global__ __launch_bounds__(kThreads) void fused_swrite_kernel(FusedArgs a, bf16* __restrict__ S,
+                                                               float* __restrict__ E) {
+  constexpr int kTile = tile_cells(FI);
+  constexpr int kRows = (FI + 2) * kPj / 4;
+  typedef Mfma<f16>::v8 h8;
+  constexpr int NK = nkc(C);
+  __shared__ __attribute__((aligned(16))) f16 is[C * kTile];
+  __shared__ __attribute__((aligned(16))) float ps[4][27 * kPsPitch];
+  __shared__ __attribute__((aligned(16))) bf16 gs[4][FI * 2][kFz];  // g of each wave's rows
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = lane >> 4, nn = lane & 15;
+  const int tz = a.n3 / kFz, tj = a.n2 / kFj;
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x);
+  const int zt = (int)(t % tz);
+  const int z0 = zt * kFz;
+  const int j0 = (int)((t / tz) % tj) * kFj;
+  const int i0 = (int)(t / tz / tj) * FI;
+  const long V = (long)a.n1 * a.n2 * a.n3;
+  const bf16* __restrict__ gsrc = (const bf16*)a.x;
+
+  const f16* kw = (const f16*)((const char*)a.apack + fused_kw_offset(C));
+  h8 Wt[2][2][NK];  // [z parity][tap half hh][channel]
+#pragma unroll
+  for (int par = 0; par < 2; ++par)
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+      for (int kc = 0; kc < NK; ++kc)
+        Wt[par][hh][kc] = *(const h8*)(kw + ((((long)par * 2 + hh) * NK + kc) * 64 + lane) * 8);
+  const float* kbias = (const float*)((const char*)kw + fused_apack_bytes(C));
+  const float b0 = kbias[lane], b1 = kbias[64 + lane];
+
+  auto row = [&](int q) {
+    const int ii = q >> 1, jj = w + 4 * (q & 1);
+    return ((long)(i0 + ii) * a.n2 + (j0 + jj)) * a.n3 + z0;
+  };
+  // g of the wave's 2 FI rows, staged with the image so the products never wait for HBM
+  {
+    bf16 gv[FI * 2];
+#pragma unroll
+    for (int q = 0; q < FI * 2; ++q) gv[q] = gsrc[row(q) + lane];
+#pragma unroll
+    for (int q = 0; q < FI * 2; ++q) gs[w][q][lane] = gv[q];
+  }
+  // image tile (FP16), zero outside the grid: the staging of the K writer
+  const f16* __restrict__ img = (const f16*)a.img;
+  {
+    constexpr unsigned kOut = 0x80000000u;
+    unsigned zi[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int z = z0 - 1 + (u == 0 ? lane : 64 + (lane & 1));
+      zi[u] = (z >= 0 && z < a.n3) ? (unsigned)z * 2u : kOut;
+    }
+    const long plane = (long)a.n2 * a.n3;
+    const int ipb = (int)(plane * 2), irb = a.n3 * 2;
+    const bool lo_i = i0 > 0, hi_i = i0 + FI < a.n1, lo_j = j0 > 0, hi_j = j0 + kFj < a.n2;
+    const long p0 = (long)(i0 - 1) * plane;
+    auto stage = [&](auto wc) {
+      constexpr int W = decltype(wc)::value;
+#pragma unroll
+      for (int k0 = 0; k0 < kRows; k0 += 13) {
+        constexpr int KB = 13;
+        f16 iv[KB][C][2];
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk) {
+          const int k = k0 + kk;
+          if (k < kRows) {
+            const int r = W + 4 * k, pi = r / kPj, pj = r % kPj;
+            const bool edge_p = pi == 0 || pi == FI + 1, edge_j = pj == 0 || pj == kPj - 1;
+            const bool pok = !edge_p || (pi == 0 ? lo_i : hi_i);
+            const bool rok = pok && (!edge_j || (pj == 0 ? lo_j : hi_j));
+            const long po = edge_p ? (pok ? (long)pi * plane : plane) : (long)pi * plane;
+            const int soff = (j0 + pj - 1);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+              const auto ir = row_rsrc(img + (c * V + p0 + po), rok ? ipb : 0);
+#pragma unroll
+              for (int u = 0; u < 2; ++u) iv[kk][c][u] = rsrc_load(ir, zi[u], soff * irb, (const f16*)nullptr);
+            }
+          }
+        }
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk) {
+          const int r = W + 4 * (k0 + kk);
+          if (k0 + kk < kRows) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + lane] = iv[kk][c][0];
+          }
+        }
+        if (lane < 2) {
+#pragma unroll
+          for (int kk = 0; kk < KB; ++kk) {
+            const int r = W + 4 * (k0 + kk);
+            if (k0 + kk < kRows) {
+#pragma unroll
+              for (int c = 0; c < C; ++c) is[c * kTile + r * kPz + 64 + lane] = iv[kk][c][1];
+            }
+          }
+        }
+      }
+    };
+    switch (w) {
+      case 0: stage(std::integral_constant<int, 0>()); break;
+      case 1: stage(std::integral_constant<int, 1>()); break;
+      case 2: stage(std::integral_constant<int, 2>()); break;
+      default: stage(std::integral_constant<int, 3>()); break;
+    }
+  }
+
+  const int o1 = col_off(fused_col(2 * g)) - 1, o2 = col_off(fused_col(2 * g + 1)) - 1;
+  const int o8 = col_off(kSplitCol) - 1;
+  const int r3e = g >= 2 ? 2 : 0, r3o = g >= 2 ? 0 : 2;
+  const int iP = o1 + 2 * nn, iQ = o2 + 2 * nn, i8e = o8 + 2 * nn + r3e, i8o = o8 + 2 * nn + r3o;
+  const int wrow = w * kPz;
+  auto lds_base = [&](int elem) {
+    unsigned b = (unsigned)(elem + wrow) * 2u;
+    asm volatile("" : "+v"(b));
+    return b;
+  };
+  const unsigned bI[6] = {lds_base(iP), lds_base(iP + 2), lds_base(iQ), lds_base(iQ + 2), lds_base(i8e),
+                          lds_base(i8o)};
+  __syncthreads();
+
+  struct Words {
+    uint32_t im[NK][6];
+  };
+  auto cb_of = [&](int q) { return (((q >> 1) + 1) * kPj + (4 * (q & 1) + 1)) * kPz + 1; };
+  auto load_words = [&](int cbq, int h) {
+    Words r;
+    const int cbyte = 2 * (cbq + 32 * h);
+#pragma unroll
+    for (int kc = 0; kc < NK; ++kc) {
+      const char* ib = (const char*)(is + kc * kTile) + cbyte;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) r.im[kc][k] = *(const uint32_t*)(ib + bI[k]);
+    }
+    return r;
+  };
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  float* __restrict__ pw = ps[w];
+  // the row-sum reads of lane z: cells z - 1, z, z + 1 (clamped into the row; the terms of
+  // cells outside the tile row are dropped below)
+  const int zm = lane > 0 ? lane - 1 : 0, zp = lane < 63 ? lane + 1 : 63;
+  const float keep_m = lane > 0 ? 1.f : 0.f, keep_p = lane < 63 ? 1.f : 0.f;
+  Words cur = load_words(cb_of(0), 0);
+#pragma unroll
+  for (int q = 0; q < FI * 2; ++q) {
+    const long vr = row(q);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int qn = h == 0 ? q : (q + 1 < FI * 2 ? q + 1 : q);
+      const Words nxt = load_words(cb_of(qn), h ^ 1);
+      // g of the lane's eight cells 32 h + 8 g .. + 7 (word i: cells 2i, 2i + 1)
+      const u32x4 gw = *(const u32x4*)&gs[w][q][32 * h + 8 * g];
+      h8 bv[2][NK];
+#pragma unroll
+      for (int kc = 0; kc < NK; ++kc) {
+        const uint32_t* m = cur.im[kc];
+        bv[0][kc] = __builtin_bit_cast(h8, u32x4{m[0], pack_lo(m[1], m[3]), m[2], m[4]});
+        bv[1][kc] = __builtin_bit_cast(h8, u32x4{m[1], pack_hi(m[0], m[2]), m[3], m[5]});
+      }
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const float bb = hh == 0 ? b0 : b1;
+        f32x4 d[2];
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+          d[par] = f32x4{bb, bb, bb, bb};
+#pragma unroll
+          for (int kc = 0; kc < NK; ++kc) d[par] = Mfma<f16>::run(bv[par][kc], Wt[par][hh][kc], d[par]);
+        }
+        // P of tap 16 hh + nn at cells 32 h + 8 g + 2 i + par
+        float pv[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          pv[2 * i] = d[0][i] * __uint_as_float(gw[i] << 16);
+          pv[2 * i + 1] = d[1][i] * __uint_as_float(gw[i] & 0xffff0000u);
+        }
+        const int p = 16 * hh + nn;
+        if (p < 27) {
+          float* dst = pw + p * kPsPitch + 32 * h + 8 * g;
+          *(f32x4*)dst = f32x4{pv[0], pv[1], pv[2], pv[3]};
+          *(f32x4*)(dst + 4) = f32x4{pv[4], pv[5], pv[6], pv[7]};
+        }
+      }
+      cur = nxt;
+    }
+    wave_sync();
+    // the nine row sums of cell z = lane; the end cells store partial sums and export the
+    // terms the neighbouring tiles miss
+    // side array of tile row rt: [end (0: cell z0, 1: cell z0 + 63)][c][2]: .x the partial
+    // sum, .y the term from the neighbouring tile, written by that tile (zero at the grid's
+    // z ends, written here)
+    const long rt = vr / kFz;  // tile row index: (grid row) * tz + zt
+    // all 27 reads before any use (no branch between them), then the stores
+    float sv[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const float* r0 = pw + 3 * (8 - c) * kPsPitch;
+      sv[c] = (keep_m * r0[2 * kPsPitch + zm] + r0[kPsPitch + lane]) + keep_p * r0[zp];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) S[c * V + vr + lane] = (bf16)sv[c];
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        const float* r0 = pw + 3 * (8 - c) * kPsPitch;
+        E[rt * 36 + 2 * c] = sv[c];  // misses P_{3c'+2} of cell z0 - 1
+        if (zt > 0) E[(rt - 1) * 36 + 18 + 2 * c + 1] = r0[0];  // the previous tile's cell z0 - 1 misses this
+        else E[rt * 36 + 2 * c + 1] = 0.f;
+      }
+    } else if (lane == 63) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) {
+        const float* r0 = pw + 3 * (8 - c) * kPsPitch;
+        E[rt * 36 + 18 + 2 * c] = sv[c];  // misses P_{3c'} of cell z0 + 64
+        if (zt + 1 < tz) E[(rt + 1) * 36 + 2 * c + 1] = r0[2 * kPsPitch + 63];  // the next tile's cell z0 + 64 misses this
+        else E[rt * 36 + 18 + 2 * c + 1] = 0.f;
+      }
+    }
+    wave_sync();
+  }
+}
+
+// dx(u) = sum_c S_c(u + (c / 3 - 1, c % 3 - 1, 0)), eight cells along z per thread; rows
+// outside the grid contribute zero.  The two end cells of a tile row take their sums from
+// the side array instead of S: the partial sum plus the term the neighbouring tile exported
+// (none at the grid's z ends).
+__global__ __launch_bounds__(kThreads) void fused_sgather_kernel(const bf16* __restrict__ S,
+                                                                 const float* __restrict__ E,
+                                                                 bf16* __restrict__ dx, int n1, int n2, int n3) {
+  const long V = (long)n1 * n2 * n3;
+  const long id = (long)xcd_block(blockIdx.x, gridDim.x) * kThreads + threadIdx.x;
+  if (id * 8 >= V) return;
+  const long v0 = id * 8;
+  const int k = (int)(v0 % n3);
+  const long rowi = v0 / n3;
+  const int j = (int)(rowi % n2), i = (int)(rowi / n2);
+  const int kt = k % kFz, tz = n3 / kFz;
+  const bool lo_end = kt == 0, hi_end = kt == kFz - 8;
+  // every lane loads its side-array pairs (branch-free; lanes inside a tile row read the
+  // pairs of the row's first cell and do not use them)
+  float2 ev[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    const int ii = min(max(i + c / 3 - 1, 0), n1 - 1), jj = min(max(j + c % 3 - 1, 0), n2 - 1);
+    const long rt = ((long)ii * n2 + jj) * tz + k / kFz;
+    ev[c] = *(const float2*)(E + rt * 36 + (hi_end ? 18 : 0) + 2 * c);
+  }
+  float acc[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) acc[t] = 0.f;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    const int ii = i + c / 3 - 1, jj = j + c % 3 - 1;
+    if (ii < 0 || ii >= n1 || jj < 0 || jj >= n2) continue;
+    float sv[8];
+    load_vec<8>(S + c * V + ((long)ii * n2 + jj) * n3 + k, sv);
+    const float ends = ev[c].x + ev[c].y;
+    sv[0] = lo_end ? ends : sv[0];
+    sv[7] = hi_end ? ends : sv[7];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] += sv[t];
+  }
+  store_vec<8>(dx + v0, acc);
+}
+
+
 // Tile depth along i: the deepest of 8, 4 planes that keeps the tile's LDS (x and C image
 // channels) within kMaxFusedLds, so at least four workgroups fit a CU, and that still
 // gives kMinFusedBlocks workgroups (about eight per CU) to spread over the chip, else 2 (at
@@ -833,6 +1121,32 @@ int launch_fused_kwrite(int C, const FusedArgs& a, void* K, hipStream_t st) {
   MLPCG_KW(1) MLPCG_KW(2) MLPCG_KW(3)
 #undef MLPCG_KW
   return check_launch("fused_kwrite");
+}
+
+size_t fused_sgrad_bytes(int n1, int n2, int n3) {
+  const size_t V = (size_t)n1 * n2 * n3;
+  return ((9 * V * 2 + 255) & ~(size_t)255) + (size_t)n1 * n2 * (n3 / kFz) * 36 * sizeof(float);
+}
+
+int launch_fused_sgrad(int C, const FusedArgs& a, void* work, void* dx, hipStream_t st) {
+  MLPCG_REQUIRE(fused_eligible(C, 1, a.n1, a.n2, a.n3), "fused_sgrad: grid (%d,%d,%d) / C=%d not supported",
+                a.n1, a.n2, a.n3, C);
+  const long V = (long)a.n1 * a.n2 * a.n3;
+  bf16* S = (bf16*)work;
+  float* E = (float*)((char*)work + ((9 * (size_t)V * 2 + 255) & ~(size_t)255));
+  const int fi = fused_fi(C, a.n1, a.n2, a.n3);
+  const unsigned nb = fused_blocks(C, a.n1, a.n2, a.n3);
+#define MLPCG_SW(CC)                                                                        \
+  if (C == CC) {                                                                            \
+    if (fi == 8) fused_swrite_kernel<CC, 8><<<nb, kThreads, 0, st>>>(a, S, E);              \
+    else if (fi == 4) fused_swrite_kernel<CC, 4><<<nb, kThreads, 0, st>>>(a, S, E);         \
+    else fused_swrite_kernel<CC, 2><<<nb, kThreads, 0, st>>>(a, S, E);                      \
+  }
+  MLPCG_SW(1) MLPCG_SW(2) MLPCG_SW(3)
+#undef MLPCG_SW
+  fused_sgather_kernel<<<(unsigned)((V / 8 + kThreads - 1) / kThreads), kThreads, 0, st>>>(S, E, (bf16*)dx, a.n1,
+                                                                                          a.n2, a.n3);
+  return check_launch("fused_sgrad");
 }
 
 int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& args, hipStream_t st) {

@@ -73,6 +73,11 @@ int launch_sm_fused(int tw, int tx, int ty, int C, const FusedArgs& a, hipStream
 // The stencils K of the fused apply's first product (a.img, a.apack, a.n1..n3), stored as
 // BF16 [27][V] (the stored-stencil layout) instead of applied.
 int launch_fused_kwrite(int C, const FusedArgs& a, void* K, hipStream_t st);
+// Input gradient of a BF16 block (no transpose) through the nine column sums S of
+// P_p(v) = g(v) K_p(v) (g in a.x, BF16; K from the first product): dx(u) = sum_c S_c(u + d_c)
+// on the matrix-core grids of the fused apply.  work: fused_sgrad_bytes() bytes.
+size_t fused_sgrad_bytes(int n1, int n2, int n3);
+int launch_fused_sgrad(int C, const FusedArgs& a, void* work, void* dx, hipStream_t st);
 
 // The two coarsest levels of the SmallSM V-cycle in one workgroup (tail.hip): from the
 // output x_in of pre[n-2] (level n-2 grid) to up(c0 x + c1 post[n-1](up(l(pool(x))))) on

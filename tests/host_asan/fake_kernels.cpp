@@ -470,6 +470,21 @@ int launch_fused_kwrite(int C, const FusedArgs& a, void* K, hipStream_t st) {
   return 0;
 }
 
+int launch_fused_sgrad(int C, const FusedArgs& a, void* work, void* dx, hipStream_t st) {
+  MLPCG_REQUIRE(fused_eligible(C, 1, a.n1, a.n2, a.n3), "fused_sgrad: not eligible");
+  const size_t V = (size_t)a.n1 * a.n2 * a.n3;
+  const size_t wb = fused_sgrad_bytes(a.n1, a.n2, a.n3);
+  fakehip::enqueue(st, [=] {
+    const char* who = "fused_sgrad";
+    RD(a.img, C * V * 2);
+    RD(a.apack, fused_weights_bytes(C));
+    RD(a.x, V * 2);
+    WR(work, wb);
+    WR(dx, V * 2);
+  });
+  return 0;
+}
+
 int launch_upsample2(int dtype, const UpArgs& a, hipStream_t st) {
   MLPCG_REQUIRE(a.n1 % 2 == 0 && a.n2 % 2 == 0 && a.n3 % 2 == 0, "upsample2: odd grid");
   const size_t V = (size_t)a.n1 * a.n2 * a.n3, s = esz(dtype);

@@ -378,20 +378,31 @@ class SmallSMModelDn3D(BaseModel):
             Y = torch.stack([y.reshape(-1) for y in ybars + ybars])
         return imgs, ybars, Y
 
-    def forward_cached(self, cache, b):
+    def linear_stack(self, dtype):
+        """The 2n SMLinear weights (rows) and biases, c0 blocks then c1, in `dtype`: the
+        operands of forward_cached's coefficient product.  A step over several scenes builds
+        them once and passes them to each scene's forward_cached, so the scenes' gradients
+        meet in one tensor before they reach the 4n parameters."""
+        lins = list(self.c0) + list(self.c1)
+        Wm = torch.stack([m.weight.reshape(-1) for m in lins]).to(dtype)
+        bv = torch.cat([m.bias for m in lins]).to(dtype)
+        return Wm, bv
+
+    def forward_cached(self, cache, b, lin=None):
         """forward(image, b) with the image pyramid and mean patches of image_cache: same
         result.  All 2n coefficients c = w . ybar + bias come from one batched product
-        (its gradients are smlinear3d.backward's: gw = g ybar, gb = g)."""
+        (its gradients are smlinear3d.backward's: gw = g ybar, gb = g); lin: the operands
+        of linear_stack(Y.dtype), if already built."""
         imgs, ybars, Y = cache
-        lins = list(self.c0) + list(self.c1)
-        Wm = torch.stack([m.weight.reshape(-1) for m in lins]).to(Y.dtype)
-        bv = torch.cat([m.bias for m in lins]).to(Y.dtype)
+        Wm, bv = lin if lin is not None else self.linear_stack(Y.dtype)
         coef = (Wm * Y).sum(1) + bv
         n = self.n
+        # one split (its backward is one concatenation of the 2n coefficient gradients,
+        # where 2n slices would each scatter into a zero vector that is then summed)
+        cs = coef.split(1)
 
         def lin(which, i):
-            k = which * n + i
-            return coef[k:k + 1]
+            return cs[which * n + i]
 
         x = [self.pre[0](imgs[0], b)]
         for i in range(1, self.n):

@@ -170,18 +170,25 @@ def residual_loss(sb: SceneBatch, x, b):
 
 
 def _step_body(model, batches, rhs, act_dtype, loss_fn):
-    """Forward + backward over the scenes; returns the mean loss as a device tensor."""
+    """Forward over the scenes, then one backward of the mean loss; returns the mean loss
+    as a device tensor.  With one backward the scenes' gradients of a parameter are summed
+    before they reach it (one accumulation per parameter, not one per scene), and the
+    scenes share one stack of the SMLinear weights."""
     total = None
+    lin = None
     for sb, b in zip(batches, rhs):
         bb = b.reshape(b.shape[0], 1, *sb.shape).to(act_dtype)
         if hasattr(model, "forward_cached"):  # the scene image is fixed: reuse its pyramid
-            x = model.forward_cached(sb.model_input(model, act_dtype), bb)
+            cache = sb.model_input(model, act_dtype)
+            if lin is None and hasattr(model, "linear_stack"):
+                lin = model.linear_stack(cache[2].dtype)
+            x = model.forward_cached(cache, bb, lin) if lin is not None else model.forward_cached(cache, bb)
         else:
             x = model(sb.image.to(act_dtype), bb)
         loss = loss_fn(sb, x, b) / len(batches)
-        loss.backward()
-        total = loss.detach() if total is None else total + loss.detach()
-    return total
+        total = loss if total is None else total + loss
+    total.backward()
+    return total.detach()
 
 
 def train_step(model, batches, rhs, optimizer, act_dtype=torch.float32, loss_fn=relative_residual_sq):

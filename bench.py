@@ -228,7 +228,8 @@ def copy_bandwidth(dev, nbytes=2 << 30):
 def train_rate(dev, steps=3, warmup=1, n=128, scenes=8, levels=5, graphed_steps=20):
     """Adam steps/s on 8 seeded 128^3 scenes (one N(0,1) RHS each), loss
     ||b - A P(b)||^2/||b||^2, hand-written backward kernels: eager steps, and the same step
-    captured in a HIP graph (fresh right-hand sides copied in before every replay)."""
+    captured in a HIP graph (right-hand sides from a pool of 4 per scene drawn before the
+    timed loop, copied in before every replay)."""
     import torch
     from mlpcg_amd import train as T
     from mlpcg_amd.sm_model_3d import SmallSMModelDn3D

@@ -547,14 +547,22 @@ __global__ __launch_bounds__(kThreads) void weight_grad_mfma_lds_kernel(
 // (loaded into registers during the previous step's products).  x rows are stored as FP32
 // (one copy; a lane's 8 taps along z are read at 4-byte alignment), image rows as three
 // BF16 copies shifted by one cell so that every B operand is one aligned 16-byte read.
-// Operand layout: A[p][k] lane 16 kb + m holds row m, voxels 8 kb .. 8 kb + 7; B[k][q]
-// lane 16 kb + m holds column m of the same voxels; D[4 kb + i][m].  Rows p = 27..31 and
-// columns q > 27C read constant zero rows, column 27C (bias) a row of ones.
+// Operand layout: A[p][k] lane 16 kb + m holds row m, voxels kv .. kv + 7 of the chunk with
+// kv = 16 (kb & 1) + 8 (kb >> 1); B[k][q] lane 16 kb + m holds column m of the same voxels;
+// D[4 kb + i][m].  Rows p = 27..31 and columns q > 27C read constant zero rows, column 27C
+// (bias) a row of ones.  LDS banks (32 x 4 bytes, 32 lanes per cycle): the two lane groups
+// of a half-wave read x words 16 apart, and the x rows of a ring (slot pitch kXP, plane
+// pitch kXPP words) start at offsets that keep the 16 taps of a half-wave on distinct
+// banks mod 16 for most ring positions (a CPU model of the x-operand reads: 107 LDS cycles
+// per step, 64 without conflicts, against 213 with the lane groups' voxels side by side and
+// 72-word rows).
 namespace wgb {
 constexpr int kSeg = 64;                       // z cells per row segment
-constexpr int kXS = 72;                        // FP32 x row: cell z0 - 1 + e at element 3 + e
+constexpr int kXS = 72;                        // FP32 zero row length; x row: cell z0 - 1 + e at element 3 + e
+constexpr int kXP = 70, kXPP = 212;            // x ring: slot and plane pitch (words)
+constexpr int kXW = 2 * kXPP + 2 * kXP + 72;   // x ring words per wave
 constexpr int kIS = 72;                        // BF16 image row copy (64 cells used)
-constexpr int kXBytes = 4 * 9 * kXS * 4;       // x ring slots of the 4 waves
+constexpr int kXBytes = 4 * kXW * 4;           // x rings of the 4 waves
 constexpr int kGBytes = 4 * kSeg * 4;          // a rows (FP32) of the 4 waves
 constexpr int kZBytes = kXS * 4;               // FP32 zero row
 constexpr int kCBytes = 2 * kIS * 2;           // BF16 zero row, ones row
@@ -586,7 +594,8 @@ __global__ __launch_bounds__(kThreads) void weight_grad_bf16_kernel(
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int m = lane & 15, kb = lane >> 4;
   // byte offsets in lds
-  const int oX = w * 9 * kXS * 4;
+  const int oX = w * kXW * 4;
+  const int kv = ((kb & 1) << 4) | ((kb >> 1) << 3);  // first voxel of the lane's 8 in a chunk
   const int oG = kXBytes + w * kSeg * 4;
   const int oZ = kXBytes + kGBytes;
   const int oC = oZ + kZBytes;
@@ -604,11 +613,11 @@ __global__ __launch_bounds__(kThreads) void weight_grad_bf16_kernel(
   for (int h = 0; h < 2; ++h) {
     const int p = 16 * h + m;
     if (p < 27) {
-      abase[h] = oX + ((p / 9) * 3 * kXS + 3 + p % 3 + 8 * kb) * 4;
+      abase[h] = oX + ((p / 9) * kXPP + 3 + p % 3 + kv) * 4;
       ae2[h] = (p / 3) % 3;
-      astr[h] = kXS * 4;
+      astr[h] = kXP * 4;
     } else {
-      abase[h] = oZ + 8 * kb * 4;
+      abase[h] = oZ + kv * 4;
       ae2[h] = 0;
       astr[h] = 0;
     }
@@ -618,11 +627,11 @@ __global__ __launch_bounds__(kThreads) void weight_grad_bf16_kernel(
     const int q = 16 * gt + m;
     if (q < 27 * C) {
       const int c = q / 27, s = q % 27;
-      bbase[gt] = oI + (((c * 9 + (s / 9) * 3) * 3 + s % 3) * kIS + 8 * kb) * 2;
+      bbase[gt] = oI + (((c * 9 + (s / 9) * 3) * 3 + s % 3) * kIS + kv) * 2;
       be2[gt] = (s / 3) % 3;
       bstr[gt] = 3 * kIS * 2;
     } else {
-      bbase[gt] = oC + ((q == 27 * C ? kIS : 0) + 8 * kb) * 2;
+      bbase[gt] = oC + ((q == 27 * C ? kIS : 0) + kv) * 2;
       be2[gt] = 0;
       bstr[gt] = 0;
     }
@@ -692,11 +701,12 @@ __global__ __launch_bounds__(kThreads) void weight_grad_bf16_kernel(
       const int t = e & 7;
       const uint32_t* d = r.d[k];
       if (e < 24) {
-        char* dst = lds + oX + (((e / 8) * 3 + sl) * kXS + 4 + 8 * t) * 4;
-        *(f4*)dst = f4{__uint_as_float(d[1] << 16), __uint_as_float(d[1] & 0xffff0000u),
-                       __uint_as_float(d[2] << 16), __uint_as_float(d[2] & 0xffff0000u)};
-        *(f4*)(dst + 16) = f4{__uint_as_float(d[3] << 16), __uint_as_float(d[3] & 0xffff0000u),
-                              __uint_as_float(d[4] << 16), __uint_as_float(d[4] & 0xffff0000u)};
+        // rows start at even words (8-byte stores)
+        char* dst = lds + oX + ((e / 8) * kXPP + sl * kXP + 4 + 8 * t) * 4;
+        typedef float f2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          *(f2*)(dst + 8 * u) = f2{__uint_as_float(d[1 + u] << 16), __uint_as_float(d[1 + u] & 0xffff0000u)};
         if (t == 0) *(float*)(dst - 4) = __uint_as_float(d[0] & 0xffff0000u);
         if (t == 7) *(float*)(dst + 32) = __uint_as_float(d[5] << 16);
       } else if (e < 24 + 24 * C) {
@@ -757,8 +767,8 @@ __global__ __launch_bounds__(kThreads) void weight_grad_bf16_kernel(
       }
 #pragma unroll
       for (int cc = 0; cc < 2; ++cc) {
-        const f4 g0 = *(const f4*)(lds + oG + (32 * cc + 8 * kb) * 4);
-        const f4 g1 = *(const f4*)(lds + oG + (32 * cc + 8 * kb + 4) * 4);
+        const f4 g0 = *(const f4*)(lds + oG + (32 * cc + kv) * 4);
+        const f4 g1 = *(const f4*)(lds + oG + (32 * cc + kv + 4) * 4);
         const float gv[8] = {g0[0], g0[1], g0[2], g0[3], g1[0], g1[1], g1[2], g1[3]};
         wg_bf16x8 ahi[2], alo[2];
 #pragma unroll

@@ -75,7 +75,7 @@ def _grads(model, sb, b, act):
 
 
 def test_train256_bf16_step(dev):
-    """graphed BF16 Adam steps on 4 x 256^3 spheres40 scenes with FP32 master
+    """The 256^3 training workload: graphed BF16 Adam steps on 4 x 256^3 spheres40 scenes with FP32 master
     weights.  The loss stays finite and falls over 20 steps on fixed right-hand sides; the
     BF16 parameter gradient is within 5e-2 (norm-wise) of the FP32 one at the same
     weights; peak HBM stays under 4 GiB, while keeping the stencils of the level-0 pre and

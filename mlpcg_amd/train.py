@@ -169,25 +169,59 @@ def residual_loss(sb: SceneBatch, x, b):
     return r.norm(dim=1).mean()
 
 
+def _block_params(model):
+    """(module, name, parameter) of the SM blocks' weights and biases: the parameters each
+    scene of a step uses once (the SMLinear weights reach the scenes through one shared
+    stack)."""
+    mods = [m for attr in ("pre", "post") for m in getattr(model, attr, [])]
+    if hasattr(model, "l"):
+        mods.append(model.l)
+    return [(m, n, m._parameters[n]) for m in mods for n in ("weight", "bias") if m._parameters.get(n) is not None]
+
+
 def _step_body(model, batches, rhs, act_dtype, loss_fn):
     """Forward over the scenes, then one backward of the mean loss; returns the mean loss
-    as a device tensor.  With one backward the scenes' gradients of a parameter are summed
-    before they reach it (one accumulation per parameter, not one per scene), and the
-    scenes share one stack of the SMLinear weights."""
+    as a device tensor.  The scenes share one stack of the SMLinear weights.  Scenes after
+    the first run the SM blocks on aliases of their parameters (same storage, separate
+    gradients), so no parameter receives one gradient per scene added one kernel at a time:
+    the aliases' gradients join the parameters' in one foreach sum per scene."""
     total = None
     lin = None
-    for sb, b in zip(batches, rhs):
+    cached = hasattr(model, "forward_cached")
+    blocks = _block_params(model) if cached else []
+    aliases = []
+    for k, (sb, b) in enumerate(zip(batches, rhs)):
         bb = b.reshape(b.shape[0], 1, *sb.shape).to(act_dtype)
-        if hasattr(model, "forward_cached"):  # the scene image is fixed: reuse its pyramid
+        if cached:  # the scene image is fixed: reuse its pyramid
             cache = sb.model_input(model, act_dtype)
             if lin is None and hasattr(model, "linear_stack"):
                 lin = model.linear_stack(cache[2].dtype)
-            x = model.forward_cached(cache, bb, lin) if lin is not None else model.forward_cached(cache, bb)
+            al = [p.detach().requires_grad_() for _, _, p in blocks] if k > 0 else []
+            for (m, n, _), a in zip(blocks, al):
+                m._parameters[n] = a
+            try:
+                x = model.forward_cached(cache, bb, lin) if lin is not None else model.forward_cached(cache, bb)
+            finally:
+                for m, n, p in blocks:
+                    m._parameters[n] = p
+            aliases.append(al)
         else:
             x = model(sb.image.to(act_dtype), bb)
         loss = loss_fn(sb, x, b) / len(batches)
         total = loss if total is None else total + loss
     total.backward()
+    for al in aliases[1:]:
+        dst, src = [], []
+        for (_, _, p), a in zip(blocks, al):
+            if a.grad is None:
+                continue
+            if p.grad is None:
+                p.grad = a.grad
+            else:
+                dst.append(p.grad)
+                src.append(a.grad)
+        if dst:
+            torch._foreach_add_(dst, src)
     return total.detach()
 
 

@@ -77,6 +77,18 @@ int mlpcg_sm_block_backward(const void* grad_out, const void* image, const void*
                             void* grad_b, void* workspace, int C, int B, int n1, int n2, int n3,
                             int dtype, void* stream);
 
+/* No counterpart in the reference: mlpcg_sm_block_forward / _backward with the FP16 copy
+ * of the image that the BF16 matrix-core path uses supplied by the caller (image16: C x V
+ * FP16, equal to the BF16 image converted; NULL = made in the workspace as by the plain
+ * entry points), for training loops that reuse a scene image over many steps.  Grids the
+ * matrix-core path does not take ignore image16. */
+int mlpcg_sm_block_forward_ex(const void* image, const void* image16, const void* x, const void* weight,
+                              const void* bias, void* y, void* workspace, int C, int B, int n1, int n2, int n3,
+                              int dtype, void* stream);
+int mlpcg_sm_block_backward_ex(const void* grad_out, const void* image, const void* image16, const void* x,
+                               const void* weight, const void* bias, void* grad_x, void* grad_w, void* grad_b,
+                               void* workspace, int C, int B, int n1, int n2, int n3, int dtype, void* stream);
+
 /* ---- SM block transpose (replaces torch_extension/sm_block_trans_3d.cpp) ----------
  * y[b,v] = sum_p K_p(v - d_p) x[b, v - d_p]. */
 /* sm_block_trans_3d.cpp:30  smblocktrans3d.inference */

@@ -8,6 +8,7 @@ for float64 data, float32 otherwise).  All work is queued on torch's current str
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import torch
 
@@ -73,7 +74,40 @@ def _block_backward(fn, g, image, x, weight, bias):
     return gx, gW.to(weight.dtype), gb.to(bias.dtype)
 
 
+# FP16 copies of BF16 images kept by callers that reuse an image over many calls (a
+# training scene's pyramid): the block entry points then skip converting the image for
+# the matrix-core path (mlpcg_sm_block_*_ex).  id(image) -> (weak reference, copy); an
+# entry goes when its image does.
+_F16_COPIES = {}
+
+
+def register_f16_image(image):
+    """Keep an FP16 copy of the BF16 image `image` for the BF16 block calls that take it
+    (the conversion the matrix-core path would otherwise make per call); returns it."""
+    if image.dtype != torch.bfloat16:
+        raise ValueError("register_f16_image: the image must be bfloat16")
+    copy = image.contiguous().to(torch.float16)
+    key = id(image)
+    _F16_COPIES[key] = (weakref.ref(image, lambda _, k=key: _F16_COPIES.pop(k, None)), copy)
+    return copy
+
+
+def _f16_copy(image, x):
+    if x.dtype != torch.bfloat16 or image.dtype != torch.bfloat16:
+        return None
+    e = _F16_COPIES.get(id(image))
+    if e is None or e[0]() is not image:
+        return None
+    c = e[1]
+    return c if c.shape == image.shape else None
+
+
 def sm_block_forward(image, x, weight, bias):
+    c16 = _f16_copy(image, x)
+    if c16 is not None:
+        fn = lambda im, *a: lib.mlpcg_sm_block_forward_ex(im, ptr(c16), *a)
+        fn.__name__ = "mlpcg_sm_block_forward_ex"
+        return _block_forward(fn, image, x, weight, bias)
     return _block_forward(lib.mlpcg_sm_block_forward, image, x, weight, bias)
 
 
@@ -82,6 +116,11 @@ def sm_block_inference(image, x, weight, bias):
 
 
 def sm_block_backward(g, image, x, weight, bias):
+    c16 = _f16_copy(image, x)
+    if c16 is not None:
+        fn = lambda gp, im, *a: lib.mlpcg_sm_block_backward_ex(gp, im, ptr(c16), *a)
+        fn.__name__ = "mlpcg_sm_block_backward_ex"
+        return _block_backward(fn, g, image, x, weight, bias)
     return _block_backward(lib.mlpcg_sm_block_backward, g, image, x, weight, bias)
 
 

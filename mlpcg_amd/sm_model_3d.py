@@ -365,13 +365,17 @@ class SmallSMModelDn3D(BaseModel):
     # -- training with a fixed scene image ------------------------------------------
     def image_cache(self, image, dtype=None):
         """The parts of forward() that depend on the image only: the pyramid (in `dtype`,
-        the activation type; forward casts the image to it) and the SMLinear mean patches
-        ybar of levels 0..n-1 (sm_linear_3d_kernel.cu:62-83).  A training loop that reuses
-        a scene image computes them once and calls forward_cached."""
+        the activation type; forward casts the image to it; BF16 pyramids also keep the
+        FP16 copies the matrix-core blocks read) and the SMLinear mean patches ybar of
+        levels 0..n-1 (sm_linear_3d_kernel.cu:62-83).  A training loop that reuses a scene
+        image computes them once and calls forward_cached."""
         with torch.no_grad():
             imgs = [image.to(dtype or image.dtype).contiguous()]
             for _ in range(self.n):
                 imgs.append(K.avgpool2(imgs[-1]))
+            if imgs[0].dtype == torch.bfloat16 and imgs[0].is_cuda:
+                for im in imgs:  # the FP16 copies the BF16 matrix-core blocks read
+                    K.register_f16_image(im)
             ybars = [K.sm_linear(imgs[i], self.c0[i].weight, self.c0[i].bias)[1]
                      for i in range(self.n)]
             # rows of the c0 (levels 0..n-1) then c1 blocks: ybar of their level

@@ -213,6 +213,24 @@ def test_sm_block_backward_bf16_dx_exact(dev, trans, C, shape):
         assert torch.equal(dx16, dx64.bfloat16())
 
 
+@pytest.mark.parametrize("C,shape", [(1, (64, 64, 64)), (2, (64, 32, 128)), (1, (8, 8, 64))])
+def test_sm_block_bf16_with_registered_f16_image(dev, C, shape):
+    """The BF16 block entry points with the caller's FP16 image copy (_ex) give the same
+    bits as the plain ones, which convert the image themselves (and grids without the
+    matrix-core path ignore the copy)."""
+    from mlpcg_amd import kernels as K
+    img = _rand((C, *shape), 70).abs().bfloat16().to(dev)
+    img_reg = img.clone()
+    K.register_f16_image(img_reg)
+    assert K._f16_copy(img_reg, img_reg) is not None and K._f16_copy(img, img) is None
+    x = _rand((1, 1, *shape), 71).bfloat16().to(dev)
+    go = _rand((1, 1, *shape), 72).bfloat16().to(dev)
+    W, b = (_rand((27, C, 3, 3, 3), 73) / 9).float().to(dev), (_rand((27,), 74) / 9).float().to(dev)
+    assert torch.equal(K.sm_block_forward(img_reg, x, W, b), K.sm_block_forward(img, x, W, b))
+    for u, v in zip(K.sm_block_backward(go, img_reg, x, W, b), K.sm_block_backward(go, img, x, W, b)):
+        assert torch.equal(u, v)
+
+
 @pytest.mark.parametrize("C,shape", [(1, (64, 64, 128)), (3, (64, 32, 192))])
 def test_sm_block_backward_bf16_dx_column_sums(dev, C, shape):
     """The block's BF16 input gradient on matrix-core grids (column sums of g K stored in

@@ -77,17 +77,23 @@ int mlpcg_sm_block_backward(const void* grad_out, const void* image, const void*
                             void* grad_b, void* workspace, int C, int B, int n1, int n2, int n3,
                             int dtype, void* stream);
 
-/* No counterpart in the reference: mlpcg_sm_block_forward / _backward with the FP16 copy
- * of the image that the BF16 matrix-core path uses supplied by the caller (image16: C x V
- * FP16, equal to the BF16 image converted; NULL = made in the workspace as by the plain
- * entry points), for training loops that reuse a scene image over many steps.  Grids the
- * matrix-core path does not take ignore image16. */
-int mlpcg_sm_block_forward_ex(const void* image, const void* image16, const void* x, const void* weight,
-                              const void* bias, void* y, void* workspace, int C, int B, int n1, int n2, int n3,
-                              int dtype, void* stream);
-int mlpcg_sm_block_backward_ex(const void* grad_out, const void* image, const void* image16, const void* x,
-                               const void* weight, const void* bias, void* grad_x, void* grad_w, void* grad_b,
-                               void* workspace, int C, int B, int n1, int n2, int n3, int dtype, void* stream);
+/* No counterpart in the reference: mlpcg_sm_block_forward / _backward with two inputs of
+ * the BF16 matrix-core path supplied by the caller, for training loops that reuse a scene
+ * image over many steps and a step's weights over its scenes: image16, the FP16 copy of
+ * the image (C x V, equal to the BF16 image converted), and wpack, the weights and bias
+ * packed by mlpcg_fused_pack (mlpcg_fused_pack_bytes(C) bytes; they must be the packing
+ * of weight and bias).  NULL = made in the workspace per call, as by the plain entry
+ * points; results are bit-identical.  Grids the matrix-core path does not take ignore
+ * both. */
+int mlpcg_sm_block_forward_ex(const void* image, const void* image16, const void* wpack, const void* x,
+                              const void* weight, const void* bias, void* y, void* workspace, int C, int B, int n1,
+                              int n2, int n3, int dtype, void* stream);
+int mlpcg_sm_block_backward_ex(const void* grad_out, const void* image, const void* image16, const void* wpack,
+                               const void* x, const void* weight, const void* bias, void* grad_x, void* grad_w,
+                               void* grad_b, void* workspace, int C, int B, int n1, int n2, int n3, int dtype,
+                               void* stream);
+size_t mlpcg_fused_pack_bytes(int C);
+int mlpcg_fused_pack(const float* weight, const float* bias, int C, void* wpack, void* stream);
 
 /* ---- SM block transpose (replaces torch_extension/sm_block_trans_3d.cpp) ----------
  * y[b,v] = sum_p K_p(v - d_p) x[b, v - d_p]. */

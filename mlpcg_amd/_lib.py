@@ -48,9 +48,11 @@ SIGNATURES = {
     "mlpcg_sm_block_forward": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
     "mlpcg_sm_block_backward": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32,
                                       i32, vp]),
-    "mlpcg_sm_block_forward_ex": (i32, [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
-    "mlpcg_sm_block_backward_ex": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32,
-                                         i32, vp]),
+    "mlpcg_sm_block_forward_ex": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),
+    "mlpcg_sm_block_backward_ex": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32,
+                                         i32, i32, vp]),
+    "mlpcg_fused_pack_bytes": (sz, [i32]),
+    "mlpcg_fused_pack": (i32, [vp, vp, i32, vp, vp]),
     "mlpcg_sm_block_trans_inference": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32,
                                              vp]),
     "mlpcg_sm_block_trans_forward": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]),

@@ -75,7 +75,8 @@ size_t mlpcg_sm_block_workspace_size(int C, int B, int n1, int n2, int n3, int d
 
 static int block_fwd(bool trans, const void* image, const void* x, const void* weight,
                      const void* bias, void* y, void* workspace, int C, int B, int n1, int n2,
-                     int n3, int dtype, void* stream, const void* image16 = nullptr) {
+                     int n3, int dtype, void* stream, const void* image16 = nullptr,
+                     const void* wpack_in = nullptr) {
   const char* who = trans ? "sm_block_trans" : "sm_block";
   if (check_block_args(who, C, B, n1, n2, n3, dtype)) return -1;
   MLPCG_REQUIRE(image && x && weight && bias && y && workspace, "%s: null pointer", who);
@@ -89,9 +90,10 @@ static int block_fwd(bool trans, const void* image, const void* x, const void* w
     // sums round as in the stored-stencil apply (K to BF16, FP32 sums).
     char* ws = (char*)workspace;
     const void* img16 = image16 ? image16 : ws;
-    void* wpack = ws + ((C * V * 2 + 255) & ~(size_t)255);
+    void* wbuf = ws + ((C * V * 2 + 255) & ~(size_t)255);
+    const void* wpack = wpack_in ? wpack_in : wbuf;
     if (!image16 && launch_cast(kBF16, kF16, image, ws, (long)(C * V), STREAM(stream))) return -1;
-    if (launch_fused_pack(kBF16, weight, bias, C, wpack, STREAM(stream))) return -1;
+    if (!wpack_in && launch_fused_pack(kBF16, weight, bias, C, wbuf, STREAM(stream))) return -1;
     FusedArgs f;
     memset(&f, 0, sizeof(f));
     f.img = img16;
@@ -119,7 +121,8 @@ static int block_fwd(bool trans, const void* image, const void* x, const void* w
 
 static int block_bwd(bool trans, const void* g, const void* image, const void* x, const void* weight,
                      const void* bias, void* gx, void* gw, void* gb, void* workspace, int C, int B,
-                     int n1, int n2, int n3, int dtype, void* stream, const void* image16 = nullptr) {
+                     int n1, int n2, int n3, int dtype, void* stream, const void* image16 = nullptr,
+                     const void* wpack_in = nullptr) {
   const char* who = trans ? "sm_block_trans_backward" : "sm_block_backward";
   if (check_block_args(who, C, B, n1, n2, n3, dtype)) return -1;
   MLPCG_REQUIRE(g && image && x && weight && bias && gx && gw && gb && workspace, "%s: null pointer", who);
@@ -137,9 +140,10 @@ static int block_bwd(bool trans, const void* g, const void* image, const void* x
   if (dtype == kBF16 && V >= kFusedForwardMinVoxels && fused_eligible(C, B, n1, n2, n3)) {
     char* ext = (char*)part + align256(weight_grad_workspace(dtype, C, B, n1, n2, n3));
     const void* img16 = image16 ? image16 : ext;
-    void* wpack = ext + align256((size_t)C * V * 2);
+    void* wbuf = ext + align256((size_t)C * V * 2);
+    const void* wpack = wpack_in ? wpack_in : wbuf;
     if (!image16 && launch_cast(kBF16, kF16, image, ext, (long)(C * V), STREAM(stream))) return -1;
-    if (launch_fused_pack(kBF16, weight, bias, C, wpack, STREAM(stream))) return -1;
+    if (!wpack_in && launch_fused_pack(kBF16, weight, bias, C, wbuf, STREAM(stream))) return -1;
     FusedArgs f;
     memset(&f, 0, sizeof(f));
     f.img = img16;
@@ -197,20 +201,32 @@ int mlpcg_sm_block_backward(const void* grad_out, const void* image, const void*
   return block_bwd(false, grad_out, image, x, weight, bias, grad_x, grad_w, grad_b, workspace, C, B, n1,
                    n2, n3, dtype, stream);
 }
-int mlpcg_sm_block_forward_ex(const void* image, const void* image16, const void* x, const void* weight,
-                              const void* bias, void* y, void* workspace, int C, int B, int n1, int n2, int n3,
-                              int dtype, void* stream) {
-  MLPCG_REQUIRE(!image16 || (dtype == kBF16 && aligned16(image16)),
-                "sm_block_forward_ex: image16 needs dtype BF16 and 16-byte alignment");
-  return block_fwd(false, image, x, weight, bias, y, workspace, C, B, n1, n2, n3, dtype, stream, image16);
+int mlpcg_sm_block_forward_ex(const void* image, const void* image16, const void* wpack, const void* x,
+                              const void* weight, const void* bias, void* y, void* workspace, int C, int B, int n1,
+                              int n2, int n3, int dtype, void* stream) {
+  MLPCG_REQUIRE((!image16 && !wpack) || dtype == kBF16, "sm_block_forward_ex: image16 / wpack need dtype BF16");
+  MLPCG_REQUIRE((!image16 || aligned16(image16)) && (!wpack || aligned16(wpack)),
+                "sm_block_forward_ex: image16 and wpack must be 16-byte aligned");
+  return block_fwd(false, image, x, weight, bias, y, workspace, C, B, n1, n2, n3, dtype, stream, image16, wpack);
 }
-int mlpcg_sm_block_backward_ex(const void* grad_out, const void* image, const void* image16, const void* x,
-                               const void* weight, const void* bias, void* grad_x, void* grad_w, void* grad_b,
-                               void* workspace, int C, int B, int n1, int n2, int n3, int dtype, void* stream) {
-  MLPCG_REQUIRE(!image16 || (dtype == kBF16 && aligned16(image16)),
-                "sm_block_backward_ex: image16 needs dtype BF16 and 16-byte alignment");
+int mlpcg_sm_block_backward_ex(const void* grad_out, const void* image, const void* image16, const void* wpack,
+                               const void* x, const void* weight, const void* bias, void* grad_x, void* grad_w,
+                               void* grad_b, void* workspace, int C, int B, int n1, int n2, int n3, int dtype,
+                               void* stream) {
+  MLPCG_REQUIRE((!image16 && !wpack) || dtype == kBF16, "sm_block_backward_ex: image16 / wpack need dtype BF16");
+  MLPCG_REQUIRE((!image16 || aligned16(image16)) && (!wpack || aligned16(wpack)),
+                "sm_block_backward_ex: image16 and wpack must be 16-byte aligned");
   return block_bwd(false, grad_out, image, x, weight, bias, grad_x, grad_w, grad_b, workspace, C, B, n1, n2, n3,
-                   dtype, stream, image16);
+                   dtype, stream, image16, wpack);
+}
+
+size_t mlpcg_fused_pack_bytes(int C) { return (C >= 1 && C <= 3) ? fused_weights_bytes(C) : 0; }
+
+int mlpcg_fused_pack(const float* weight, const float* bias, int C, void* wpack, void* stream) {
+  MLPCG_REQUIRE(weight && bias && wpack, "fused_pack: null pointer");
+  MLPCG_REQUIRE(C >= 1 && C <= 3, "fused_pack: C must be 1, 2 or 3 (got %d)", C);
+  MLPCG_REQUIRE(aligned16(wpack), "fused_pack: wpack must be 16-byte aligned");
+  return launch_fused_pack(kBF16, weight, bias, C, wpack, STREAM(stream));
 }
 int mlpcg_sm_block_trans_inference(const void* image, const void* x, const void* weight,
                                    const void* bias, void* y, void* workspace, int C, int B, int n1,

@@ -7,6 +7,7 @@ for float64 data, float32 otherwise).  All work is queued on torch's current str
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import weakref
 
@@ -102,10 +103,55 @@ def _f16_copy(image, x):
     return c if c.shape == image.shape else None
 
 
-def sm_block_forward(image, x, weight, bias):
+def fused_pack(weight, bias):
+    """The weights and bias of a BF16 block packed for the matrix-core path (uint8 device
+    tensor of mlpcg_fused_pack_bytes(C) bytes)."""
+    require_cuda(weight, bias)
+    W, b = weight.detach().float().contiguous(), bias.detach().float().contiguous()
+    Cimg = int(W.shape[1])
+    out = torch.empty(lib.mlpcg_fused_pack_bytes(Cimg), dtype=torch.uint8, device=W.device)
+    check(lib.mlpcg_fused_pack(ptr(W), ptr(b), Cimg, ptr(out), stream_ptr(W.device)), "fused_pack")
+    return out
+
+
+# Packings made once for a stretch of calls in which the weights do not change (one
+# training step over several scenes), keyed by the (weight, bias) storage.
+_PACKS = {}
+
+
+@contextlib.contextmanager
+def packed_weights(pairs):
+    """Inside the block, BF16 block calls on these (weight, bias) pairs -- or on tensors
+    sharing their storage -- use one packing made on entry; the weights must not change
+    until the block ends."""
+    keys = []
+    try:
+        for weight, bias in pairs:
+            W, b = weight.detach().float().contiguous(), bias.detach().float().contiguous()
+            key = (W.data_ptr(), b.data_ptr())
+            _PACKS[key] = fused_pack(W, b)
+            keys.append(key)
+        yield
+    finally:
+        for key in keys:
+            _PACKS.pop(key, None)
+
+
+def _ex_args(image, x, weight, bias):
+    """(image16, wpack) for the _ex block entry points, or None when neither is kept."""
+    if x.dtype != torch.bfloat16:
+        return None
     c16 = _f16_copy(image, x)
-    if c16 is not None:
-        fn = lambda im, *a: lib.mlpcg_sm_block_forward_ex(im, ptr(c16), *a)
+    pk = _PACKS.get((weight.data_ptr(), bias.data_ptr())) if weight.dtype == torch.float32 else None
+    if c16 is None and pk is None:
+        return None
+    return (ptr(c16) if c16 is not None else None, ptr(pk) if pk is not None else None)
+
+
+def sm_block_forward(image, x, weight, bias):
+    ex = _ex_args(image, x, weight, bias)
+    if ex is not None:
+        fn = lambda im, *a: lib.mlpcg_sm_block_forward_ex(im, ex[0], ex[1], *a)
         fn.__name__ = "mlpcg_sm_block_forward_ex"
         return _block_forward(fn, image, x, weight, bias)
     return _block_forward(lib.mlpcg_sm_block_forward, image, x, weight, bias)
@@ -116,9 +162,9 @@ def sm_block_inference(image, x, weight, bias):
 
 
 def sm_block_backward(g, image, x, weight, bias):
-    c16 = _f16_copy(image, x)
-    if c16 is not None:
-        fn = lambda gp, im, *a: lib.mlpcg_sm_block_backward_ex(gp, im, ptr(c16), *a)
+    ex = _ex_args(image, x, weight, bias)
+    if ex is not None:
+        fn = lambda gp, im, *a: lib.mlpcg_sm_block_backward_ex(gp, im, ex[0], ex[1], *a)
         fn.__name__ = "mlpcg_sm_block_backward_ex"
         return _block_backward(fn, g, image, x, weight, bias)
     return _block_backward(lib.mlpcg_sm_block_backward, g, image, x, weight, bias)

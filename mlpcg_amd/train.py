@@ -184,11 +184,21 @@ def _step_body(model, batches, rhs, act_dtype, loss_fn):
     as a device tensor.  The scenes share one stack of the SMLinear weights.  Scenes after
     the first run the SM blocks on aliases of their parameters (same storage, separate
     gradients), so no parameter receives one gradient per scene added one kernel at a time:
-    the aliases' gradients join the parameters' in one foreach sum per scene."""
-    total = None
-    lin = None
+    the aliases' gradients join the parameters' in one foreach sum per scene.  With BF16
+    activations the blocks' weights are packed for the matrix cores once per step."""
     cached = hasattr(model, "forward_cached")
     blocks = _block_params(model) if cached else []
+    if cached and act_dtype == torch.bfloat16:
+        # the matrix-core blocks' weight packing, once per step instead of per call
+        mods = list(dict.fromkeys(m for m, _, _ in blocks))
+        with K.packed_weights([(m.weight, m.bias) for m in mods]):
+            return _scenes_step(model, batches, rhs, act_dtype, loss_fn, cached, blocks)
+    return _scenes_step(model, batches, rhs, act_dtype, loss_fn, cached, blocks)
+
+
+def _scenes_step(model, batches, rhs, act_dtype, loss_fn, cached, blocks):
+    total = None
+    lin = None
     aliases = []
     for k, (sb, b) in enumerate(zip(batches, rhs)):
         bb = b.reshape(b.shape[0], 1, *sb.shape).to(act_dtype)

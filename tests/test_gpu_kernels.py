@@ -215,9 +215,9 @@ def test_sm_block_backward_bf16_dx_exact(dev, trans, C, shape):
 
 @pytest.mark.parametrize("C,shape", [(1, (64, 64, 64)), (2, (64, 32, 128)), (1, (8, 8, 64))])
 def test_sm_block_bf16_with_registered_f16_image(dev, C, shape):
-    """The BF16 block entry points with the caller's FP16 image copy (_ex) give the same
-    bits as the plain ones, which convert the image themselves (and grids without the
-    matrix-core path ignore the copy)."""
+    """The BF16 block entry points with the caller's FP16 image copy and packed weights
+    (_ex) give the same bits as the plain ones, which convert the image and pack the
+    weights themselves (and grids without the matrix-core path ignore both)."""
     from mlpcg_amd import kernels as K
     img = _rand((C, *shape), 70).abs().bfloat16().to(dev)
     img_reg = img.clone()
@@ -229,6 +229,15 @@ def test_sm_block_bf16_with_registered_f16_image(dev, C, shape):
     assert torch.equal(K.sm_block_forward(img_reg, x, W, b), K.sm_block_forward(img, x, W, b))
     for u, v in zip(K.sm_block_backward(go, img_reg, x, W, b), K.sm_block_backward(go, img, x, W, b)):
         assert torch.equal(u, v)
+    # and with the weights packed once for a stretch of calls (a view sharing the storage
+    # finds the packing; none is kept after the block)
+    y_ref, bw_ref = K.sm_block_forward(img, x, W, b), K.sm_block_backward(go, img, x, W, b)
+    with K.packed_weights([(W, b)]):
+        assert K._ex_args(img, x, W.view_as(W), b)[1] is not None
+        assert torch.equal(K.sm_block_forward(img_reg, x, W, b), y_ref)
+        for u, v in zip(K.sm_block_backward(go, img, x, W, b), bw_ref):
+            assert torch.equal(u, v)
+    assert K._ex_args(img, x, W, b) is None
 
 
 @pytest.mark.parametrize("C,shape", [(1, (64, 64, 128)), (3, (64, 32, 192))])

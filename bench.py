@@ -245,7 +245,7 @@ def train_rate(dev, steps=3, warmup=1, n=128, scenes=8, levels=5, graphed_steps=
     eager = steps / t
     torch.manual_seed(1)
     model = SmallSMModelDn3D(levels, 1, "trilinear").to(dev)
-    opt = torch.optim.Adam(model.parameters(), lr=1e-4, capturable=True)
+    opt = torch.optim.Adam(model.parameters(), lr=1e-4, capturable=True, fused=True)
     step = T.GraphedTrainStep(model, batches, rhs, opt)
     pool = [[T.white_rhs(sb, 1, gen) for sb in batches] for _ in range(4)]
     step(pool[0])
@@ -364,7 +364,7 @@ def train256_steps(args, dev, scenes, act_name, steps, gen):
     act = {"bf16": torch.bfloat16, "fp32": torch.float32}[act_name]
     model, wsrc = load_model(args.weights, args.levels, dev)
     model.train()
-    opt = torch.optim.Adam(model.parameters(), lr=3e-4, capturable=True)
+    opt = torch.optim.Adam(model.parameters(), lr=3e-4, capturable=True, fused=True)
     pool = [[T.white_rhs(sb, 1, gen) for sb in scenes] for _ in range(min(steps, TRAIN256_POOL))]
     pool_bytes = sum(b.numel() * b.element_size() for rhs in pool for b in rhs)
     torch.cuda.reset_peak_memory_stats(dev)

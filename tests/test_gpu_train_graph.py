@@ -68,12 +68,13 @@ def test_dot_loss_matches_torch_loss_and_gradient(dev, act, gtol):
     assert float((grads[0] - grads[1]).norm() / grads[0].norm()) < gtol
 
 
-@pytest.mark.parametrize("act", [torch.float32, torch.bfloat16])
-def test_graphed_step_matches_eager_steps(dev, act):
+@pytest.mark.parametrize("act,fused", [(torch.float32, False), (torch.bfloat16, False), (torch.bfloat16, True)])
+def test_graphed_step_matches_eager_steps(dev, act, fused):
+    """fused: torch's fused Adam (one kernel for all parameters), as the benchmarks use."""
     T, model, sbs, gen, B = _setup(dev)
     model_g = copy.deepcopy(model)
-    opt_e = torch.optim.Adam(model.parameters(), lr=3e-3, capturable=True)
-    opt_g = torch.optim.Adam(model_g.parameters(), lr=3e-3, capturable=True)
+    opt_e = torch.optim.Adam(model.parameters(), lr=3e-3, capturable=True, fused=fused)
+    opt_g = torch.optim.Adam(model_g.parameters(), lr=3e-3, capturable=True, fused=fused)
     steps = [[T.white_rhs(sb, B, gen) for sb in sbs] for _ in range(4)]
     graphed = T.GraphedTrainStep(model_g, sbs, steps[0], opt_g, act_dtype=act)
     for p, q in zip(model.parameters(), model_g.parameters()):

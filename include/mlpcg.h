@@ -190,12 +190,13 @@ int mlpcg_loss_grad(const float* r, const uint8_t* code, const double* coef, voi
  * there (sm_model_3d.py:245,258,282,286).  y = c0 x0 + c1 x1 over n elements of dtype
  * MLPCG_F32 or MLPCG_BF16, with torch's roundings (coefficients, of c_dtype MLPCG_F32 or
  * MLPCG_BF16, rounded to dtype; each product and the sum rounded to dtype); its backward
- * g0 = c0 g, g1 = c1 g (g0 or g1 may be NULL) and out2 = [g.x0, g.x1] accumulated in FP64
- * (workspace: mlpcg_dot_workspace_size() bytes). */
+ * g0 = c0 g, g1 = c1 g (g0 or g1 may be NULL) and out2 = [g.x0, g.x1] accumulated in FP64,
+ * stored as out_dtype (MLPCG_F64 or MLPCG_F32; workspace: mlpcg_dot_workspace_size() bytes). */
 int mlpcg_comb(const void* x0, const void* x1, const void* c0, const void* c1, int c_dtype, void* y, long n,
                int dtype, void* stream);
 int mlpcg_comb_backward(const void* g, const void* x0, const void* x1, const void* c0, const void* c1, int c_dtype,
-                        void* g0, void* g1, long n, int dtype, double* out2, void* workspace, void* stream);
+                        void* g0, void* g1, long n, int dtype, void* out2, int out_dtype, void* workspace,
+                        void* stream);
 /* y = A x, CSR with int32 indices; dtype MLPCG_F32 or MLPCG_F64 (val, x, y alike). */
 int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const void* x, void* y,
                    long n, int dtype, void* stream);

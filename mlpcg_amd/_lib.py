@@ -73,7 +73,7 @@ SIGNATURES = {
     "mlpcg_flags_code": (i32, [vp, vp, i32, i32, i32, vp]),
     "mlpcg_spmv_flags": (i32, [vp, vp, vp, i32, i32, i32, vp]),
     "mlpcg_comb": (i32, [vp, vp, vp, vp, i32, vp, i64, i32, vp]),
-    "mlpcg_comb_backward": (i32, [vp, vp, vp, vp, vp, i32, vp, vp, i64, i32, vp, vp, vp]),
+    "mlpcg_comb_backward": (i32, [vp, vp, vp, vp, vp, i32, vp, vp, i64, i32, vp, i32, vp, vp]),
     "mlpcg_loss_resid": (i32, [vp, i32, vp, vp, vp, i32, i32, i32, vp, vp, vp]),
     "mlpcg_loss_grad": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, vp]),
     "mlpcg_spmv_csr": (i32, [vp, vp, vp, vp, vp, i64, i32, vp]),

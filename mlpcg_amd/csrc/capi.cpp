@@ -385,11 +385,13 @@ int mlpcg_comb(const void* x0, const void* x1, const void* c0, const void* c1, i
 }
 
 int mlpcg_comb_backward(const void* g, const void* x0, const void* x1, const void* c0, const void* c1, int c_dtype,
-                        void* g0, void* g1, long n, int dtype, double* out2, void* workspace, void* stream) {
+                        void* g0, void* g1, long n, int dtype, void* out2, int out_dtype, void* workspace,
+                        void* stream) {
   MLPCG_REQUIRE(n >= 0, "comb_backward: bad length %ld", n);
   MLPCG_REQUIRE(c0 && c1 && out2 && workspace && (n == 0 || (g && x0 && x1)), "comb_backward: null pointer");
   MLPCG_REQUIRE(aligned16(workspace), "comb_backward: workspace must be 16-byte aligned");
-  return launch_comb_grad(dtype, g, x0, x1, c0, c1, c_dtype, g0, g1, n, (double*)workspace, out2, STREAM(stream));
+  return launch_comb_grad(dtype, g, x0, x1, c0, c1, c_dtype, g0, g1, n, (double*)workspace, out2, out_dtype,
+                          STREAM(stream));
 }
 
 int mlpcg_spmv_csr(const int* rowptr, const int* col, const void* val, const void* x, void* y, long n,

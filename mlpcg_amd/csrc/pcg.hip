@@ -671,7 +671,9 @@ __global__ __launch_bounds__(kThreads) void npsd_dir_csr_kernel(CsrArgs a, const
 // Finalisation of the two-kernel reductions: one block sums the partials in block order
 // and updates the solver scalars.
 // ---------------------------------------------------------------------------
-enum FinOp : int { kFinResidInit = 0, kFinResid, kFinPW, kFinRZ, kFinDot, kFinProj, kFinNpsd, kFinRZ2, kFinDot2 };
+enum FinOp : int {
+  kFinResidInit = 0, kFinResid, kFinPW, kFinRZ, kFinDot, kFinProj, kFinNpsd, kFinRZ2, kFinDot2, kFinDot2F32
+};
 
 // aux/aux2: the output of kFinDot; alfa of N and O for kFinProj; alfa of the new
 // direction for kFinNpsd.
@@ -679,7 +681,8 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
                                                                const double* __restrict__ partials,
                                                                unsigned nb, SolveState* st,
                                                                double* aux, const double* aux2) {
-  if (op != kFinResidInit && op != kFinDot && op != kFinDot2 && st->done) return;  // uniform over the block
+  if (op != kFinResidInit && op != kFinDot && op != kFinDot2 && op != kFinDot2F32 && st->done)
+    return;  // uniform over the block
   // thread 0 fetches the scalars it will read while the block sums the partials
   double rho = 0.0, rho_old = 0.0, norm_b = 0.0, tol = 0.0, atol = 0.0, cross = 0.0, an = 0.0, ao = 0.0;
   int iter = 0, max_it = 0, method = 0;
@@ -746,6 +749,10 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int nr,
     case kFinDot2:
       aux[0] = t[0];
       aux[1] = t[1];
+      break;
+    case kFinDot2F32:  // the two sums rounded to FP32 (aux points at two floats)
+      reinterpret_cast<float*>(aux)[0] = (float)t[0];
+      reinterpret_cast<float*>(aux)[1] = (float)t[1];
       break;
     case kFinProj: {
       const double gN = t[0] / an;
@@ -973,8 +980,9 @@ int launch_comb(int dt, const void* x0, const void* x1, const void* c0, const vo
 }
 
 int launch_comb_grad(int dt, const void* g, const void* x0, const void* x1, const void* c0, const void* c1, int ct,
-                     void* g0, void* g1, long n, double* partials, double* out2, hipStream_t st) {
+                     void* g0, void* g1, long n, double* partials, void* out2, int out_dt, hipStream_t st) {
   MLPCG_REQUIRE(ct == kF32 || ct == kBF16, "comb_grad: coefficients must be FP32 or BF16 (got %d)", ct);
+  MLPCG_REQUIRE(out_dt == kF64 || out_dt == kF32, "comb_grad: the sums are FP64 or FP32 (got %d)", out_dt);
   const int vz = comb_vz(dt, n, {g, x0, x1, g0, g1});
   const long nch = n / vz;
   const unsigned nb = red_blocks(nch);
@@ -989,7 +997,7 @@ int launch_comb_grad(int dt, const void* g, const void* x0, const void* x1, cons
     MLPCG_REQUIRE(false, "comb_grad: elements must be FP32 or BF16 (got %d)", dt);
   }
 #undef MLPCG_CG
-  finalize(kFinDot2, 2, partials, nb, nullptr, out2, st);
+  finalize(out_dt == kF32 ? kFinDot2F32 : kFinDot2, 2, partials, nb, nullptr, (double*)out2, st);
   return check_launch("comb_grad");
 }
 

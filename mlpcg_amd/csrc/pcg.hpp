@@ -96,11 +96,11 @@ int launch_loss_grad(int tg, const float* r, const uint8_t* code, const double* 
                      int n3, hipStream_t st);
 // Level combination y = c0 x0 + c1 x1 (FP32 or BF16 elements, coefficients of type ct
 // rounded to the element type first) and its backward g0 = c0 g, g1 = c1 g (either may be
-// null), out2 = [g.x0, g.x1] in FP64.
+// null), out2 = [g.x0, g.x1] accumulated in FP64, stored as out_dt (FP64 or FP32).
 int launch_comb(int dt, const void* x0, const void* x1, const void* c0, const void* c1, int ct, void* y, long n,
                 hipStream_t st);
 int launch_comb_grad(int dt, const void* g, const void* x0, const void* x1, const void* c0, const void* c1, int ct,
-                     void* g0, void* g1, long n, double* partials, double* out2, hipStream_t st);
+                     void* g0, void* g1, long n, double* partials, void* out2, int out_dt, hipStream_t st);
 int launch_dot(const float* x, const float* y, long n, double* partials,
                double* out, hipStream_t st);
 

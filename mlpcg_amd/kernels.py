@@ -344,8 +344,9 @@ def comb(c0, x0, c1, x1):
     return y
 
 
-def comb_backward(g, c0, x0, c1, x1, need_g0=True, need_g1=True):
-    """Backward of comb: (c0 * g or None, c1 * g or None, (2,) float64 [g.x0, g.x1])."""
+def comb_backward(g, c0, x0, c1, x1, need_g0=True, need_g1=True, out_dtype=torch.float64):
+    """Backward of comb: (c0 * g or None, c1 * g or None, (2,) [g.x0, g.x1] accumulated in
+    float64 and returned as out_dtype (float64 or float32))."""
     require_cuda(g, c0, x0, c1, x1)
     g, x0, x1 = g.contiguous(), x0.contiguous(), x1.contiguous()
     if not (g.shape == x0.shape == x1.shape and g.dtype == x0.dtype == x1.dtype):
@@ -353,10 +354,10 @@ def comb_backward(g, c0, x0, c1, x1, need_g0=True, need_g1=True):
     c0, c1 = _coefs(c0, c1)
     g0 = torch.empty_like(g) if need_g0 else None
     g1 = torch.empty_like(g) if need_g1 else None
-    out = torch.empty(2, dtype=torch.float64, device=g.device)
+    out = torch.empty(2, dtype=out_dtype, device=g.device)
     check(lib.mlpcg_comb_backward(ptr(g), ptr(x0), ptr(x1), ptr(c0), ptr(c1), code_of(c0.dtype),
                                   ptr(g0) if need_g0 else None, ptr(g1) if need_g1 else None, g.numel(),
-                                  code_of(g.dtype), ptr(out), ptr(_dot_workspace(g.device)),
+                                  code_of(g.dtype), ptr(out), code_of(out_dtype), ptr(_dot_workspace(g.device)),
                                   stream_ptr(g.device)), "comb_backward")
     return g0, g1, out
 

@@ -249,7 +249,9 @@ class _CombFunction(torch.autograd.Function):
         c0, x0, c1, x1 = ctx.saved_tensors
         need = ctx.needs_input_grad
         if _comb_native(c0, x0, c1, x1) and g.dtype == x0.dtype:
-            g0, g1, dots = K.comb_backward(g, c0, x0, c1, x1, need[1], need[3])
+            # the coefficient gradients straight in the coefficients' type when that is FP32
+            odt = torch.float32 if c0.dtype == c1.dtype == torch.float32 else torch.float64
+            g0, g1, dots = K.comb_backward(g, c0, x0, c1, x1, need[1], need[3], out_dtype=odt)
             return (dots[0:1].reshape(c0.shape).to(c0.dtype) if need[0] else None, g0,
                     dots[1:2].reshape(c1.shape).to(c1.dtype) if need[2] else None, g1)
 

@@ -43,9 +43,11 @@ def test_comb_kernels_match_torch(dev, dt, cdt, n, off):
     ref = torch.stack([(gr.double() * x0.double()).sum(), (gr.double() * x1.double()).sum()])
     scale = torch.stack([(gr.double() * x0.double()).abs().sum(), (gr.double() * x1.double()).abs().sum()])
     assert bool(((dots - ref).abs() <= 1e-13 * scale + 1e-300).all())
-    # either input gradient may be skipped
+    # either input gradient may be skipped; the sums may be returned rounded to FP32
     h0, h1, d2 = K.comb_backward(gr, c0, x0, c1, x1, need_g0=False)
     assert h0 is None and torch.equal(h1, g1) and torch.equal(d2, dots)
+    _, _, d32 = K.comb_backward(gr, c0, x0, c1, x1, out_dtype=torch.float32)
+    assert d32.dtype == torch.float32 and torch.equal(d32, dots.float())
 
 
 @pytest.mark.parametrize("act,gtol", [(torch.float32, 1e-5), (torch.bfloat16, 8e-3)])

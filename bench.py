@@ -25,7 +25,8 @@ stored-stencil plan and the FP32 plan (--skip-c3 leaves it out).
 train: Adam steps/s of the graph-captured training step on 8 x 128^3 scenes (FP32), and
 train.train256 that of the 4 x 256^3 training step (40-sphere scenes, shipped weights,
 right-hand sides from a pool resident in HBM) with BF16 and with FP32 activations, and
-their ratio (--skip-train leaves both out).
+their ratio, for the one-channel model and (three_channel) the three-channel one
+(--skip-train leaves both out).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -343,15 +344,15 @@ def workload_spheres128(args, dev):
     print(json.dumps(out), flush=True)
 
 
-def train256_scenes(dev):
+def train256_scenes(dev, num_imgs=1):
     from mlpcg_amd import train as T
-    return [T.SceneBatch(s, 1, dev) for s in T.make_scenes(256, range(300, 304), "spheres40")]
+    return [T.SceneBatch(s, num_imgs, dev) for s in T.make_scenes(256, range(300, 304), "spheres40")]
 
 
 TRAIN256_POOL = 16  # right-hand sides per scene drawn before the timed loop
 
 
-def train256_steps(args, dev, scenes, act_name, steps, gen):
+def train256_steps(args, dev, scenes, act_name, steps, gen, weights=None, num_imgs=1):
     """`steps` Adam steps on the 4 training scenes at 256^3 from the shipped weights, each
     step one HIP-graph replay with BF16 or FP32 activations (FP32 master weights).  The
     right-hand sides are a dataset resident in HBM, as the reference's training set is
@@ -362,7 +363,7 @@ def train256_steps(args, dev, scenes, act_name, steps, gen):
     import torch
     from mlpcg_amd import train as T
     act = {"bf16": torch.bfloat16, "fp32": torch.float32}[act_name]
-    model, wsrc = load_model(args.weights, args.levels, dev)
+    model, wsrc = load_model(weights or args.weights, args.levels, dev, num_imgs=num_imgs)
     model.train()
     opt = torch.optim.Adam(model.parameters(), lr=3e-4, capturable=True, fused=True)
     pool = [[T.white_rhs(sb, 1, gen) for sb in scenes] for _ in range(min(steps, TRAIN256_POOL))]
@@ -383,7 +384,8 @@ def train256_steps(args, dev, scenes, act_name, steps, gen):
 
 def train256_rate(args, dev, steps=40):
     """The 256^3 training step of the train256 workload, BF16 vs FP32 activations
-    (steps/s after the graph capture, no held-out solves)."""
+    (steps/s after the graph capture, no held-out solves), with the one-channel shipped
+    weights and (three_channel, unless --skip-c3) the three-channel ones."""
     import torch
     scenes = train256_scenes(dev)
     gen = torch.Generator(device=dev).manual_seed(3)
@@ -395,6 +397,16 @@ def train256_rate(args, dev, steps=40):
         _, sps, loss, peak, _ = train256_steps(args, dev, scenes, name, steps, gen)
         out[name] = {"steps_per_s": round(sps, 3), "peak_hbm_GiB": round(peak, 2), "last_loss": loss}
     out["bf16_vs_fp32"] = round(out["bf16"]["steps_per_s"] / out["fp32"]["steps_per_s"], 3)
+    if not args.skip_c3 and os.path.exists(C3_WEIGHTS):
+        # the reference's three-channel model ([air, fluid, non-solid] images) on the same scenes
+        del scenes
+        scenes3 = train256_scenes(dev, 3)
+        c3 = {"weights": os.path.relpath(C3_WEIGHTS, ROOT)}
+        for name in ("bf16", "fp32"):
+            _, sps, loss, peak, _ = train256_steps(args, dev, scenes3, name, steps, gen, C3_WEIGHTS, 3)
+            c3[name] = {"steps_per_s": round(sps, 3), "peak_hbm_GiB": round(peak, 2), "last_loss": loss}
+        c3["bf16_vs_fp32"] = round(c3["bf16"]["steps_per_s"] / c3["fp32"]["steps_per_s"], 3)
+        out["three_channel"] = c3
     return out
 
 
@@ -595,6 +607,10 @@ def main():
         train["train256"] = t256
         log(f"[bench] train256: bf16 {t256['bf16']['steps_per_s']:.2f} steps/s, fp32 "
             f"{t256['fp32']['steps_per_s']:.2f} steps/s ({t256['bf16_vs_fp32']:.2f}x)")
+        if "three_channel" in t256:
+            t3 = t256["three_channel"]
+            log(f"[bench] train256 three channels: bf16 {t3['bf16']['steps_per_s']:.2f} steps/s, fp32 "
+                f"{t3['fp32']['steps_per_s']:.2f} steps/s ({t3['bf16_vs_fp32']:.2f}x)")
 
     cpu = None
     if not args.skip_cpu:
